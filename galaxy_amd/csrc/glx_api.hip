@@ -1,0 +1,581 @@
+// C ABI of the LBVH gravity step (include/galaxy_lbvh.h). Every entry point converts
+// runtime failures into a GLX_* status plus a thread-local message.
+#include <math.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/galaxy_lbvh.h"
+#include "glx_runtime.h"
+
+using namespace glx;
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return GLX_OK;
+  } catch (const Error& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return GLX_ENOMEM;
+  } catch (...) {
+    g_last_error = "unexpected internal error";
+    return GLX_EDEVICE;
+  }
+}
+
+extern "C" const char* glx_last_error(void) { return g_last_error.c_str(); }
+
+static constexpr size_t kMaxN = (size_t)1 << 29;
+
+static void validate_params(const glx_params* p) {
+  if (!p) fail(GLX_EINVAL, "params is NULL");
+  if (!isfinite(p->g)) fail(GLX_EINVAL, "g must be finite");
+  if (!isfinite(p->softening) || p->softening < 0.0) fail(GLX_EINVAL, "softening must be >= 0");
+  if (!isfinite(p->theta) || !(p->theta > 0.0)) fail(GLX_EINVAL, "theta must be > 0");
+  if (p->leaf_size > 1) fail(GLX_EINVAL, "leaf_size must be 1");
+  if (p->multipole != 0) fail(GLX_EINVAL, "multipole must be 0 (monopole)");
+  if (p->precision > GLX_PREC_KAHAN) fail(GLX_EINVAL, "unknown precision mode");
+}
+
+static void validate_n(size_t n) {
+  if (n > kMaxN) fail(GLX_EINVAL, "n exceeds 2^29 particles per system");
+}
+
+static void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
+  d.n = n;
+  d.g = (float)p.g;
+  d.eps2 = (float)(p.softening * p.softening);
+  d.theta = (float)p.theta;
+  d.precision = p.precision;
+}
+
+static hipStream_t make_stream() {
+  hipStream_t st = nullptr;
+  check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  return st;
+}
+
+namespace {
+// RAII device buffer for one-shot calls.
+template <class T>
+struct DevArray {
+  T* p = nullptr;
+  explicit DevArray(size_t n) {
+    if (n) check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+  }
+  ~DevArray() { (void)hipFree(p); }
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+};
+
+struct Stream {
+  hipStream_t st;
+  Stream() : st(make_stream()) {}
+  ~Stream() { (void)hipStreamDestroy(st); }
+};
+
+void h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "upload");
+}
+void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "download");
+}
+void sync(hipStream_t st) { check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+}  // namespace
+
+extern "C" int glx_device_info(char* name, size_t name_len, int* compute_units,
+                               uint64_t* hbm_bytes) {
+  return guard([&] {
+    ensure_device();
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+    if (name && name_len) {
+      strncpy(name, prop.name, name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (uint64_t)prop.totalGlobalMem;
+  });
+}
+
+// ----------------------------------------------------------------------------------
+// ForceSolver
+// ----------------------------------------------------------------------------------
+
+struct glx_solver {
+  glx_params p{};
+  hipStream_t st = nullptr;
+  Segment seg;
+  Batch* batch = nullptr;
+  uint32_t n_cur = 0xFFFFFFFFu;
+  uint64_t stats[2] = {0, 0};
+  double* pot_partial = nullptr;
+  size_t pot_cap = 0;
+  ~glx_solver() {
+    delete batch;
+    (void)hipFree(pot_partial);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" int glx_solver_create(const glx_params* params, glx_solver** out) {
+  return guard([&] {
+    if (!out) fail(GLX_EINVAL, "out is NULL");
+    *out = nullptr;
+    validate_params(params);
+    ensure_device();
+    glx_solver* s = new glx_solver();
+    try {
+      s->p = *params;
+      s->st = make_stream();
+      s->batch = new Batch(s->st);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+extern "C" int glx_solver_destroy(glx_solver* s) {
+  return guard([&] { delete s; });
+}
+
+// Upload positions/masses into the solver's segment and (re)describe it when its
+// size or storage changed.
+static void solver_stage_inputs(glx_solver* s, size_t n, const double* pos, const double* mass) {
+  const uint32_t cap_before = s->seg.capacity;
+  s->seg.reserve((uint32_t)n, true);
+  fill_params(s->seg.desc, s->p, (uint32_t)n);
+  if (s->n_cur != n || s->seg.capacity != cap_before) {
+    s->batch->set_segments({s->seg.desc});
+    s->n_cur = (uint32_t)n;
+  }
+  h2d(s->seg.desc.x, pos, 3 * n * sizeof(double), s->st);
+  h2d(s->seg.desc.m, mass, n * sizeof(double), s->st);
+}
+
+extern "C" int glx_solver_accelerations(glx_solver* s, size_t n, const double* pos,
+                                        const double* mass, double* acc) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "solver is NULL");
+    validate_n(n);
+    if (n == 0) return;
+    if (!pos || !mass || !acc) fail(GLX_EINVAL, "NULL array");
+    s->stats[0] = s->stats[1] = 0;
+    if (n == 1) {
+      acc[0] = acc[1] = acc[2] = 0.0;
+      return;
+    }
+    solver_stage_inputs(s, n, pos, mass);
+    const bool count = (s->p.flags & GLX_FLAG_COUNT) != 0;
+    if (count) check(hipMemsetAsync(s->seg.desc.counters, 0, 32, s->st), "memset counters");
+    Batch& b = *s->batch;
+    b.reset_bbox();
+    b.bbox();
+    b.build_and_walk(0, s->p.precision == GLX_PREC_KAHAN, count);
+    d2h(acc, s->seg.desc.acc_out, 3 * n * sizeof(double), s->st);
+    unsigned long long c[2] = {0, 0};
+    if (count) d2h(c, s->seg.desc.counters, sizeof(c), s->st);
+    sync(s->st);
+    s->stats[0] = c[0];
+    s->stats[1] = c[1];
+  });
+}
+
+extern "C" int glx_solver_accelerations_active(glx_solver* s, size_t n, const double* pos,
+                                               const double* mass, const size_t* active,
+                                               size_t n_active, double* acc) {
+  (void)active;
+  (void)n_active;
+  return glx_solver_accelerations(s, n, pos, mass, acc);
+}
+
+extern "C" int glx_solver_stats(glx_solver* s, uint64_t* interactions, uint64_t* node_visits) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "solver is NULL");
+    if (interactions) *interactions = s->stats[0];
+    if (node_visits) *node_visits = s->stats[1];
+  });
+}
+
+extern "C" int glx_solver_potential_energy(glx_solver* s, size_t n, const double* pos,
+                                           const double* mass, double* out) {
+  return guard([&] {
+    if (!s || !out) fail(GLX_EINVAL, "NULL argument");
+    validate_n(n);
+    *out = 0.0;
+    if (n < 2) return;
+    if (!pos || !mass) fail(GLX_EINVAL, "NULL array");
+    solver_stage_inputs(s, n, pos, mass);
+    const size_t nb = (n + 255) / 256 + 1;
+    if (nb > s->pot_cap) {
+      (void)hipFree(s->pot_partial);
+      s->pot_partial = nullptr;
+      check(hipMalloc(&s->pot_partial, nb * sizeof(double)), "hipMalloc(potential)");
+      s->pot_cap = nb;
+    }
+    check(launch_potential_f64(s->st, s->seg.desc.x, s->seg.desc.m, (uint32_t)n, s->p.g,
+                               s->p.softening * s->p.softening, s->pot_partial,
+                               s->pot_partial + (nb - 1)),
+          "potential");
+    d2h(out, s->pot_partial + (nb - 1), sizeof(double), s->st);
+    sync(s->st);
+  });
+}
+
+extern "C" int glx_direct_sum_f64(size_t n, const double* pos, const double* mass, double g,
+                                  double softening, const uint32_t* targets, size_t n_targets,
+                                  double* acc_out) {
+  return guard([&] {
+    validate_n(n);
+    if (n_targets == 0) return;
+    if (!pos || !mass || !targets || !acc_out) fail(GLX_EINVAL, "NULL array");
+    for (size_t t = 0; t < n_targets; ++t)
+      if (targets[t] >= n) fail(GLX_EINVAL, "target index out of range");
+    ensure_device();
+    Stream S;
+    DevArray<double> x(3 * n), m(n), out(3 * n_targets);
+    DevArray<uint32_t> tg(n_targets);
+    h2d(x.p, pos, 3 * n * sizeof(double), S.st);
+    h2d(m.p, mass, n * sizeof(double), S.st);
+    h2d(tg.p, targets, n_targets * sizeof(uint32_t), S.st);
+    check(launch_direct_f64(S.st, x.p, m.p, (uint32_t)n, tg.p, (uint32_t)n_targets, g,
+                            softening * softening, out.p),
+          "direct_f64");
+    d2h(acc_out, out.p, 3 * n_targets * sizeof(double), S.st);
+    sync(S.st);
+  });
+}
+
+// ----------------------------------------------------------------------------------
+// stage exports
+// ----------------------------------------------------------------------------------
+
+namespace {
+struct StageCtx {
+  Stream S;
+  Segment seg;
+  Batch batch;
+  StageCtx() : batch(S.st) {}
+  void describe(uint32_t n) {
+    seg.reserve(n, false);
+    glx_params p{};
+    p.g = 1.0;
+    p.softening = 0.0;
+    p.theta = 1.0;
+    fill_params(seg.desc, p, n);
+    batch.set_segments({seg.desc});
+  }
+};
+}  // namespace
+
+extern "C" int glx_stage_morton(size_t n, const double* pos, double* bounds_out, uint32_t* lanes,
+                                uint32_t* codes) {
+  return guard([&] {
+    validate_n(n);
+    if (n == 0) fail(GLX_EINVAL, "reference_morton requires a non-empty system");
+    if (!pos || !codes) fail(GLX_EINVAL, "NULL array");
+    ensure_device();
+    StageCtx c;
+    c.describe((uint32_t)n);
+    DevArray<uint32_t> dl(lanes ? 3 * n : 0);
+    h2d(c.seg.desc.x, pos, 3 * n * sizeof(double), c.S.st);
+    c.batch.reset_bbox();
+    c.batch.bbox();
+    c.batch.encode(dl.p);
+    d2h(codes, c.seg.desc.keys[0], n * sizeof(uint32_t), c.S.st);
+    if (lanes) d2h(lanes, dl.p, 3 * n * sizeof(uint32_t), c.S.st);
+    double b[8];
+    d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
+    sync(c.S.st);
+    if (bounds_out) memcpy(bounds_out, b, 5 * sizeof(double));
+  });
+}
+
+extern "C" int glx_stage_sort(size_t n, const uint32_t* codes, uint32_t* order,
+                              uint32_t* sorted_codes) {
+  return guard([&] {
+    validate_n(n);
+    if (n == 0) return;
+    if (!codes) fail(GLX_EINVAL, "NULL array");
+    ensure_device();
+    StageCtx c;
+    c.describe((uint32_t)n);
+    h2d(c.seg.desc.keys[0], codes, n * sizeof(uint32_t), c.S.st);
+    c.batch.iota();
+    c.batch.sort();
+    if (order) d2h(order, c.seg.desc.vals[0], n * sizeof(uint32_t), c.S.st);
+    if (sorted_codes) d2h(sorted_codes, c.seg.desc.keys[0], n * sizeof(uint32_t), c.S.st);
+    sync(c.S.st);
+  });
+}
+
+extern "C" int glx_stage_karras(size_t n, const uint32_t* sorted_codes, uint32_t* left,
+                                uint32_t* right, uint32_t* parent) {
+  return guard([&] {
+    validate_n(n);
+    if (n == 0) fail(GLX_EINVAL, "reference_karras requires a non-empty system");
+    if (!sorted_codes || !parent || (n > 1 && (!left || !right))) fail(GLX_EINVAL, "NULL array");
+    for (size_t k = 1; k < n; ++k)
+      if (sorted_codes[k - 1] > sorted_codes[k]) fail(GLX_EINVAL, "codes must be sorted");
+    if (n == 1) {
+      parent[0] = kNoParent;
+      return;
+    }
+    ensure_device();
+    StageCtx c;
+    c.describe((uint32_t)n);
+    h2d(c.seg.desc.keys[0], sorted_codes, n * sizeof(uint32_t), c.S.st);
+    c.batch.karras();
+    d2h(left, c.seg.desc.left, (n - 1) * sizeof(uint32_t), c.S.st);
+    d2h(right, c.seg.desc.right, (n - 1) * sizeof(uint32_t), c.S.st);
+    d2h(parent, c.seg.desc.parent_leaf, n * sizeof(uint32_t), c.S.st);
+    d2h(parent + n, c.seg.desc.parent_int, (n - 1) * sizeof(uint32_t), c.S.st);
+    sync(c.S.st);
+  });
+}
+
+extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const double* sorted_pos,
+                                   const double* sorted_mass, double* origin_out, float* aabb_min,
+                                   float* aabb_max, float* com, float* mass) {
+  return guard([&] {
+    validate_n(n);
+    if (n == 0) fail(GLX_EINVAL, "reference_aggregate requires a non-empty system");
+    if (!sorted_codes || !sorted_pos || !sorted_mass || !aabb_min || !aabb_max || !com || !mass)
+      fail(GLX_EINVAL, "NULL array");
+    for (size_t k = 1; k < n; ++k)
+      if (sorted_codes[k - 1] > sorted_codes[k]) fail(GLX_EINVAL, "codes must be sorted");
+    ensure_device();
+    StageCtx c;
+    c.describe((uint32_t)n);
+    h2d(c.seg.desc.x, sorted_pos, 3 * n * sizeof(double), c.S.st);
+    h2d(c.seg.desc.m, sorted_mass, n * sizeof(double), c.S.st);
+    h2d(c.seg.desc.keys[0], sorted_codes, n * sizeof(uint32_t), c.S.st);
+    c.batch.reset_bbox();
+    c.batch.bbox();
+    c.batch.bounds_only();
+    c.batch.iota();
+    c.batch.gather();
+    c.batch.karras();
+    c.batch.aggregate();
+    std::vector<float4> lf(n);
+    std::vector<Node> nd(n > 1 ? n - 1 : 0);
+    double b[8];
+    d2h(lf.data(), c.seg.desc.leaf, n * sizeof(float4), c.S.st);
+    if (n > 1) d2h(nd.data(), c.seg.desc.node, (n - 1) * sizeof(Node), c.S.st);
+    d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
+    sync(c.S.st);
+    if (origin_out) memcpy(origin_out, b + 5, 3 * sizeof(double));
+    for (size_t k = 0; k < n; ++k) {
+      const float p[3] = {lf[k].x, lf[k].y, lf[k].z};
+      for (int a = 0; a < 3; ++a) aabb_min[3 * k + a] = aabb_max[3 * k + a] = com[3 * k + a] = p[a];
+      mass[k] = lf[k].w;
+    }
+    for (size_t j = 0; j + 1 < n; ++j) {
+      const size_t u = n + j;
+      const Node& q = nd[j];
+      const float cm[3] = {q.a.x, q.a.y, q.a.z}, mn[3] = {q.b.x, q.b.y, q.b.z},
+                  mx[3] = {q.c.x, q.c.y, q.c.z};
+      for (int a = 0; a < 3; ++a) {
+        com[3 * u + a] = cm[a];
+        aabb_min[3 * u + a] = mn[a];
+        aabb_max[3 * u + a] = mx[a];
+      }
+      mass[u] = q.a.w;
+    }
+  });
+}
+
+// ----------------------------------------------------------------------------------
+// resident stepper
+// ----------------------------------------------------------------------------------
+
+struct glx_sys {
+  glx_params p{};
+  hipStream_t st = nullptr;
+  Segment seg;
+  Batch* batch = nullptr;
+  uint32_t n = 0;
+  double time = 0.0;
+  double cur_dt = NAN;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~glx_sys() {
+    drop_graph();
+    delete batch;
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" int glx_sys_create(const glx_params* params, glx_sys** out) {
+  return guard([&] {
+    if (!out) fail(GLX_EINVAL, "out is NULL");
+    *out = nullptr;
+    validate_params(params);
+    ensure_device();
+    glx_sys* s = new glx_sys();
+    try {
+      s->p = *params;
+      s->st = make_stream();
+      s->batch = new Batch(s->st);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+extern "C" int glx_sys_destroy(glx_sys* s) {
+  return guard([&] { delete s; });
+}
+
+extern "C" int glx_sys_upload(glx_sys* s, size_t n, const double* pos, const double* vel,
+                              const double* mass) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "sys is NULL");
+    validate_n(n);
+    s->drop_graph();
+    s->time = 0.0;
+    s->n = 0;
+    s->cur_dt = NAN;
+    if (n == 0) return;
+    if (!pos || !vel || !mass) fail(GLX_EINVAL, "NULL array");
+    s->seg.reserve((uint32_t)n, false);
+    fill_params(s->seg.desc, s->p, (uint32_t)n);
+    s->seg.desc.dt = 0.0;
+    s->seg.desc.half_dt = 0.0f;
+    s->batch->set_segments({s->seg.desc});
+    std::vector<float> v32(3 * n);
+    for (size_t k = 0; k < 3 * n; ++k) v32[k] = (float)vel[k];
+    h2d(s->seg.desc.x, pos, 3 * n * sizeof(double), s->st);
+    h2d(s->seg.desc.v, v32.data(), 3 * n * sizeof(float), s->st);
+    h2d(s->seg.desc.m, mass, n * sizeof(double), s->st);
+    Batch& b = *s->batch;
+    b.reset_bbox();
+    b.bbox();
+    b.build_and_walk(0, s->p.precision == GLX_PREC_KAHAN, false);  // prime a(x0)
+    sync(s->st);
+    s->n = (uint32_t)n;
+  });
+}
+
+extern "C" int glx_sys_step(glx_sys* s, double dt, uint64_t k) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "sys is NULL");
+    if (!isfinite(dt)) fail(GLX_EINVAL, "dt must be finite");
+    if (s->n == 0) {
+      s->time += (double)k * dt;
+      return;
+    }
+    if (k == 0) return;
+    if (!(dt == s->cur_dt)) {
+      std::vector<SegDesc> d = s->batch->host_descs();
+      d[0].dt = dt;
+      d[0].half_dt = (float)(0.5 * dt);
+      s->batch->set_segments(d);
+      s->cur_dt = dt;
+    }
+    if (!s->exec) {
+      check(hipStreamBeginCapture(s->st, hipStreamCaptureModeRelaxed), "BeginCapture");
+      try {
+        s->batch->kick_drift_bbox();
+        s->batch->build_and_walk(1, s->p.precision == GLX_PREC_KAHAN, false);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s->st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      check(hipStreamEndCapture(s->st, &s->graph), "EndCapture");
+      check(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0), "GraphInstantiate");
+    }
+    for (uint64_t q = 0; q < k; ++q) {
+      check(hipGraphLaunch(s->exec, s->st), "GraphLaunch");
+      s->time += dt;
+    }
+    sync(s->st);
+  });
+}
+
+extern "C" int glx_sys_snapshot(glx_sys* s, double* pos, double* vel) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "sys is NULL");
+    const size_t n = s->n;
+    if (n == 0) return;
+    std::vector<float> v32(vel ? 3 * n : 0);
+    if (pos) d2h(pos, s->seg.desc.x, 3 * n * sizeof(double), s->st);
+    if (vel) d2h(v32.data(), s->seg.desc.v, 3 * n * sizeof(float), s->st);
+    sync(s->st);
+    if (vel)
+      for (size_t q = 0; q < 3 * n; ++q) vel[q] = (double)v32[q];
+  });
+}
+
+extern "C" int glx_sys_snapshot_accel(glx_sys* s, double* acc) {
+  return guard([&] {
+    if (!s || !acc) fail(GLX_EINVAL, "NULL argument");
+    const size_t n = s->n;
+    if (n == 0) return;
+    std::vector<float> a32(3 * n);
+    d2h(a32.data(), s->seg.desc.acc, 3 * n * sizeof(float), s->st);
+    sync(s->st);
+    for (size_t q = 0; q < 3 * n; ++q) acc[q] = (double)a32[q];
+  });
+}
+
+extern "C" int glx_sys_time(glx_sys* s, double* t) {
+  return guard([&] {
+    if (!s || !t) fail(GLX_EINVAL, "NULL argument");
+    *t = s->time;
+  });
+}
+
+extern "C" int glx_sys_len(glx_sys* s, size_t* n) {
+  return guard([&] {
+    if (!s || !n) fail(GLX_EINVAL, "NULL argument");
+    *n = s->n;
+  });
+}
+
+extern "C" int glx_sys_diag(glx_sys* s, int with_potential, glx_diag* out) {
+  return guard([&] {
+    if (!s || !out) fail(GLX_EINVAL, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    if (s->n == 0) return;
+    double r[kDiagRecord];
+    s->batch->diag(r);
+    out->kinetic = r[0];
+    for (int a = 0; a < 3; ++a) {
+      out->momentum[a] = r[1 + a];
+      out->angular_momentum[a] = r[4 + a];
+      out->center_of_mass[a] = r[7 + a] / r[10];
+    }
+    out->mass = r[10];
+    if (with_potential && s->n > 1) {
+      const size_t nb = ((size_t)s->n + 255) / 256 + 1;
+      DevArray<double> part(nb);
+      check(launch_potential_f64(s->st, s->seg.desc.x, s->seg.desc.m, s->n, s->p.g,
+                                 s->p.softening * s->p.softening, part.p, part.p + (nb - 1)),
+            "potential");
+      d2h(&out->potential, part.p + (nb - 1), sizeof(double), s->st);
+      sync(s->st);
+    }
+  });
+}

@@ -1,0 +1,75 @@
+// Shared device/host definitions of the MI355X LBVH gravity step.
+//
+// Memory model: every system ("segment") owns its own device arrays, listed in a
+// SegDesc that lives in device memory; a batched launch covers many segments by
+// iterating over a table of Tile descriptors {segment, first element, count}. A
+// single system is the one-segment case of the same kernels.
+//
+// Per-segment index spaces (all local to the segment):
+//   particles        i in [0, n)          original order (the caller's index order)
+//   sorted leaves    k in [0, n)          Morton order; leaf k holds particle vals[k]
+//   internal nodes   j in [0, n-1)        Karras numbering; unified index n + j
+//   unified index    u < n  -> leaf u,    u >= n -> internal node u - n, root = n
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace glx {
+
+constexpr uint32_t kNoParent = 0xFFFFFFFFu;  // solvers/src/lbvh.rs:244
+constexpr uint32_t kEnd = 0xFFFFFFFFu;       // walk terminator
+constexpr uint32_t kMortonBits = 10;         // solvers/src/lbvh.rs:32
+constexpr int kRadixBits = 8;
+constexpr int kRadixBins = 1 << kRadixBits;
+constexpr int kRadixPasses = 4;              // 30-bit codes, 8-bit digits
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
+constexpr int kBlock = 256;                   // threads per block of element kernels
+
+// Walk record of one internal node (64 B, one aligned half cache line).
+struct __align__(16) Node {
+  float4 a;  // centre of mass (origin-relative f32), mass
+  float4 b;  // AABB min (origin-relative f32), squared acceptance radius
+  float4 c;  // AABB max (origin-relative f32), left child (unified index, bits)
+  uint4 d;   // next node after this subtree (unified or kEnd), spare x3
+};
+
+// Precision of the walk's accumulation.
+enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1 };
+
+struct SegDesc {
+  // resident state (original particle order)
+  double* x;        // 3n positions, f64, interleaved xyz
+  float* v;         // 3n velocities, f32
+  double* m;        // n masses, f64
+  float* acc;       // 3n accelerations (f32, already x G) cached between steps
+  double* acc_out;  // optional 3n f64 output of a one-shot force evaluation
+  // build scratch
+  uint32_t* keys[2];
+  uint32_t* vals[2];
+  float4* leaf;          // n sorted leaves: origin-relative f32 position, mass
+  Node* node;            // n-1 internal-node walk records
+  uint32_t* left;        // n-1 left child (unified)
+  uint32_t* right;       // n-1 right child (unified)
+  uint32_t* parent_leaf; // n parent of leaf k (unified)
+  uint32_t* parent_int;  // n-1 parent of internal node j (unified)
+  uint32_t* rs;          // n-1 right child of the internal node whose split is s
+  uint32_t* hi;          // n-1 last leaf of internal node j's range
+  uint32_t* flag;        // n-1 arrival counters of the bottom-up pass (kept at 0)
+  unsigned long long* bbox;  // 6 order-preserving encodings: min xyz, max xyz
+  double* bounds;        // 8: bmin xyz, size, scale, origin xyz
+  unsigned long long* counters;  // 4 walk statistics (interactions, node visits)
+  uint32_t n;
+  uint32_t sort_tile0;   // first sort tile of this segment in the tile tables
+  uint32_t sort_tiles;   // number of sort tiles
+  uint32_t precision;
+  float g, eps2, theta, half_dt;
+  double dt;
+};
+
+struct Tile {
+  uint32_t seg, start, count, pad;
+};
+
+}  // namespace glx

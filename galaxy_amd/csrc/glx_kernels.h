@@ -1,0 +1,32 @@
+// Launch wrappers of the device kernels (internal to the library).
+#pragma once
+#include "glx_common.h"
+
+namespace glx {
+
+constexpr int kDiagRecord = 11;  // KE, P xyz, L xyz, sum m*x xyz, sum m
+
+hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg);
+hipError_t launch_bbox(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_kick_drift_bbox(hipStream_t st, const SegDesc* segs, const Tile* et,
+                                  uint32_t net);
+hipError_t launch_encode(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net,
+                         uint32_t* lanes_out);
+hipError_t launch_bounds_only(hipStream_t st, const SegDesc* segs, uint32_t nseg);
+hipError_t launch_iota(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_sort(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
+                       uint32_t net, uint32_t* tile_hist, uint32_t* tile_off);
+hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_karras(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_aggregate(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_walk(hipStream_t st, const SegDesc* segs, const Tile* wt, uint32_t nwt,
+                       int mode, bool kahan, bool count);
+hipError_t launch_diag(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
+                       uint32_t net, double* partial, double* out);
+hipError_t launch_direct_f64(hipStream_t st, const double* x, const double* m, uint32_t n,
+                             const uint32_t* targets, uint32_t nt, double g, double eps2,
+                             double* out);
+hipError_t launch_potential_f64(hipStream_t st, const double* x, const double* m, uint32_t n,
+                                double g, double eps2, double* partial, double* out);
+
+}  // namespace glx

@@ -1,0 +1,95 @@
+// Host runtime of the LBVH gravity step: device segments, batches of segments with
+// their tile tables, and the launch sequences of one force evaluation / KDK step.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "glx_kernels.h"
+
+namespace glx {
+
+// Failure inside the runtime: carries a GLX_* status and a message. Never crosses
+// the C ABI (every entry point catches it).
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void check(hipError_t e, const char* what);
+void ensure_device();
+
+// One system's device arrays (a single allocation carved into the arrays of
+// SegDesc). Capacity grows on demand and never shrinks.
+class Segment {
+ public:
+  Segment() = default;
+  ~Segment();
+  Segment(const Segment&) = delete;
+  Segment& operator=(const Segment&) = delete;
+
+  // Ensure room for n particles; with_acc_out adds the f64 force output buffer.
+  void reserve(uint32_t n, bool with_acc_out);
+  static size_t bytes_for(uint32_t n, bool with_acc_out);
+  size_t bytes() const { return bytes_; }
+
+  SegDesc desc{};  // host copy of the descriptor (pointers into the allocation)
+  uint32_t capacity = 0;
+  bool has_acc_out = false;
+
+ private:
+  void* block_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// A set of segments launched together, with the tile tables and the scratch shared
+// by one launch sequence.
+class Batch {
+ public:
+  explicit Batch(hipStream_t st) : stream_(st) {}
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  // Rebuild tables for the given descriptors (copied to the device).
+  void set_segments(const std::vector<SegDesc>& descs);
+  uint32_t nseg() const { return nseg_; }
+  const std::vector<SegDesc>& host_descs() const { return host_; }
+  SegDesc* dev_descs() const { return segs_; }
+
+  // Launch sequences (all on the batch stream, no host synchronization).
+  void reset_bbox();
+  void bbox();
+  void kick_drift_bbox();
+  void encode(uint32_t* lanes_out = nullptr);
+  void bounds_only();
+  void iota();
+  void sort();
+  void gather();
+  void karras();
+  void aggregate();
+  void walk(int mode, bool kahan, bool count);
+  // bbox-independent tail: encode .. walk (bbox must already be accumulated)
+  void build_and_walk(int mode, bool kahan, bool count);
+  void diag(double* host_out /* nseg * kDiagRecord */);
+
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  hipStream_t stream_;
+  uint32_t nseg_ = 0;
+  std::vector<SegDesc> host_;
+  SegDesc* segs_ = nullptr;
+  Tile* etiles_ = nullptr;
+  Tile* wtiles_ = nullptr;
+  uint32_t n_et_ = 0, n_wt_ = 0;
+  uint32_t* tile_hist_ = nullptr;
+  uint32_t* tile_off_ = nullptr;
+  double* diag_partial_ = nullptr;
+  double* diag_out_ = nullptr;
+  size_t cap_segs_ = 0, cap_et_ = 0, cap_wt_ = 0;
+};
+
+}  // namespace glx

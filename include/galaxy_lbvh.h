@@ -1,0 +1,167 @@
+/*
+ * galaxy_lbvh.h — C ABI of the MI355X-native LBVH gravity step.
+ *
+ * Drop-in boundary for the reference's LBVH path (BoykoNeov/galaxy). Each entry point
+ * names the reference interface it replaces (file:line in the reference tree). Plain
+ * pointers and sizes only; no C++ exceptions cross this boundary; every call returns
+ * a status code and, on failure, leaves a message readable by glx_last_error().
+ *
+ * Layouts: positions, velocities and accelerations are interleaved xyz f64 arrays of
+ * 3n doubles (the memory layout of a Rust &[DVec3]); masses are n doubles.
+ *
+ * Threading: a handle is used by one thread at a time (the reference's &mut self);
+ * distinct handles may be used from distinct threads — each owns its own HIP stream.
+ */
+#ifndef GALAXY_LBVH_H
+#define GALAXY_LBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GLX_OK 0
+#define GLX_EINVAL 1    /* invalid argument (precondition the reference asserts) */
+#define GLX_ENOMEM 2    /* device or host allocation failed */
+#define GLX_EDEVICE 3   /* no usable GPU, or a HIP call failed */
+#define GLX_EBUDGET 4   /* pool admission: device-memory budget exhausted */
+#define GLX_ENOSLOT 5   /* pool admission: no free job slot */
+#define GLX_ENOTFOUND 6 /* unknown job id */
+
+/* walk accumulation precision */
+#define GLX_PREC_F32 0   /* plain f32 accumulation */
+#define GLX_PREC_KAHAN 1 /* f32 with Kahan-compensated accumulation */
+
+/* flags */
+#define GLX_FLAG_COUNT 1u /* count walk interactions / node visits (glx_solver_stats) */
+
+/* Solver parameters: the (g, softening, theta) of Lbvh::new
+ * (solvers/src/lbvh.rs:652-659) plus this implementation's knobs. */
+typedef struct glx_params {
+  double g;           /* gravitational constant */
+  double softening;   /* Plummer softening length eps (> = 0) */
+  double theta;       /* opening angle (> 0) */
+  uint32_t leaf_size; /* bodies per leaf; must be 1 (the reference's LBVH leaf) */
+  uint32_t multipole; /* 0 = monopole (the reference's expansion) */
+  uint32_t precision; /* GLX_PREC_* */
+  uint32_t flags;     /* GLX_FLAG_* */
+} glx_params;
+
+/* FP64 conservation diagnostics (core/src/diagnostics.rs:8-68). */
+typedef struct glx_diag {
+  double kinetic;            /* 0.5 sum m v^2 */
+  double potential;          /* softened potential (if requested), else 0 */
+  double momentum[3];        /* sum m v */
+  double angular_momentum[3];/* sum m r x v */
+  double center_of_mass[3];  /* sum m r / sum m */
+  double mass;               /* sum m */
+} glx_diag;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *glx_last_error(void);
+
+/* Device name, compute-unit count and HBM bytes of device 0 (GLX_EDEVICE if none). */
+int glx_device_info(char *name, size_t name_len, int *compute_units, uint64_t *hbm_bytes);
+
+/* ---------------------------------------------------------------------------------
+ * ForceSolver (core/src/traits.rs:24-27,52-54,95) as implemented by Lbvh
+ * (solvers/src/lbvh.rs:643-683) and GpuLbvhFused (gpu/src/gpu_lbvh_fused.rs:50-134).
+ * ------------------------------------------------------------------------------- */
+typedef struct glx_solver glx_solver;
+
+/* Lbvh::new / GpuLbvhFused::new (solvers/src/lbvh.rs:653; gpu/src/gpu_lbvh_fused.rs:50).
+ * GLX_EINVAL for theta <= 0, softening < 0, non-finite g/eps/theta, leaf_size != 1,
+ * multipole != 0; GLX_EDEVICE without a GPU. */
+int glx_solver_create(const glx_params *params, glx_solver **out);
+int glx_solver_destroy(glx_solver *solver);
+
+/* ForceSolver::accelerations (core/src/traits.rs:27; solvers/src/lbvh.rs:663-676;
+ * gpu/src/gpu_lbvh_fused.rs:57-134). acc receives 3n doubles. n == 0 is a no-op and
+ * n == 1 yields zero acceleration (gpu/src/gpu_lbvh.rs:410-418). Device buffers grow
+ * on demand and never shrink (gpu/src/fused_core.rs:432). */
+int glx_solver_accelerations(glx_solver *solver, size_t n, const double *pos,
+                             const double *mass, double *acc);
+
+/* ForceSolver::accelerations_active default (core/src/traits.rs:52-54): computes
+ * the full pass; entries of acc outside `active` are filled as well. */
+int glx_solver_accelerations_active(glx_solver *solver, size_t n, const double *pos,
+                                    const double *mass, const size_t *active,
+                                    size_t n_active, double *acc);
+
+/* ForceSolver::potential_energy (core/src/traits.rs:95; solvers/src/lbvh.rs:680-682):
+ * exact O(n^2) FP64 softened potential, computed on the GPU. */
+int glx_solver_potential_energy(glx_solver *solver, size_t n, const double *pos,
+                                const double *mass, double *out);
+
+/* Walk statistics of the last accelerations call (needs GLX_FLAG_COUNT): evaluated
+ * terms (accepted nodes + leaf bodies other than the target) and node visits. No
+ * reference counterpart (measurement hook). */
+int glx_solver_stats(glx_solver *solver, uint64_t *interactions, uint64_t *node_visits);
+
+/* Exact FP64 direct sum (solvers/src/direct_sum.rs:25-45 pair kernel) onto a subset
+ * of targets against all n sources, self excluded; acc_out holds 3 * n_targets. The
+ * FP64 error reference for sampled targets at sizes where O(n^2) on the CPU is too
+ * slow. */
+int glx_direct_sum_f64(size_t n, const double *pos, const double *mass, double g,
+                       double softening, const uint32_t *targets, size_t n_targets,
+                       double *acc_out);
+
+/* ---------------------------------------------------------------------------------
+ * Stage exports: the GPU counterparts of the reference's stage oracles, which it makes
+ * public for exactly this gating (solvers/src/lib.rs:15-18).
+ * ------------------------------------------------------------------------------- */
+
+/* reference_morton (solvers/src/lbvh.rs:105-129): bounds_out = {bmin xyz, size, scale};
+ * lanes (3n, may be NULL) and codes (n). Bit-exact with the reference (f64). n > 0. */
+int glx_stage_morton(size_t n, const double *pos, double *bounds_out, uint32_t *lanes,
+                     uint32_t *codes);
+
+/* reference_sort (solvers/src/lbvh.rs:143-147): order sorting by (code, index), and
+ * the sorted codes. Codes must be < 2^32 (30-bit Morton codes in practice). */
+int glx_stage_sort(size_t n, const uint32_t *codes, uint32_t *order, uint32_t *sorted_codes);
+
+/* reference_karras (solvers/src/lbvh.rs:276-306): left/right (n-1) and parent
+ * (2n-1, unified index; parent[root] = 0xFFFFFFFF). n > 0. */
+int glx_stage_karras(size_t n, const uint32_t *sorted_codes, uint32_t *left, uint32_t *right,
+                     uint32_t *parent);
+
+/* reference_aggregate (solvers/src/lbvh.rs:329-379) as computed by the device: per
+ * unified node (2n-1) AABB min/max, centre of mass (3 floats each) and mass, in f32
+ * coordinates relative to origin_out (the root cube centre of the sorted positions,
+ * f64). n > 0. */
+int glx_stage_aggregate(size_t n, const uint32_t *sorted_codes, const double *sorted_pos,
+                        const double *sorted_mass, double *origin_out, float *aabb_min,
+                        float *aabb_max, float *com, float *mass);
+
+/* ---------------------------------------------------------------------------------
+ * Resident stepper: GpuResidentLeapfrog (gpu/src/gpu_resident.rs:1246-1968). State
+ * stays on the device between upload and snapshot; one KDK step is one hipGraph
+ * launch. Positions are FP64 on the device, velocities and forces FP32.
+ * ------------------------------------------------------------------------------- */
+typedef struct glx_sys glx_sys;
+
+/* GpuResidentLeapfrog::new (gpu/src/gpu_resident.rs:1292). */
+int glx_sys_create(const glx_params *params, glx_sys **out);
+int glx_sys_destroy(glx_sys *sys);
+/* upload (gpu/src/gpu_resident.rs:1685): copies state, primes a(x0), resets time. */
+int glx_sys_upload(glx_sys *sys, size_t n, const double *pos, const double *vel,
+                   const double *mass);
+/* step / step_many (gpu/src/gpu_resident.rs:1823,1848): k KDK steps of dt. */
+int glx_sys_step(glx_sys *sys, double dt, uint64_t k);
+/* snapshot (gpu/src/gpu_resident.rs:1878): pos/vel (3n each, either may be NULL). */
+int glx_sys_snapshot(glx_sys *sys, double *pos, double *vel);
+/* snapshot_accel (gpu/src/gpu_resident.rs:1655): cached a (3n doubles). */
+int glx_sys_snapshot_accel(glx_sys *sys, double *acc);
+/* time / len (gpu/src/gpu_resident.rs:1947,1960). */
+int glx_sys_time(glx_sys *sys, double *time);
+int glx_sys_len(glx_sys *sys, size_t *n);
+/* FP64 device diagnostics; with_potential adds the exact O(n^2) FP64 potential. */
+int glx_sys_diag(glx_sys *sys, int with_potential, glx_diag *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GALAXY_LBVH_H */

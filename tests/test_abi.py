@@ -1,0 +1,72 @@
+"""CPU tests of the C ABI boundary: the library loads, exports every symbol the
+public header declares, validates arguments before touching a device, and reports a
+missing GPU as an error (never a silent fallback). No compute calls."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import galaxy_amd
+from galaxy_amd import lbvh as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "galaxy_lbvh.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(glx_\w+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("glx_solver_create", "glx_solver_accelerations", "glx_stage_morton",
+                 "glx_stage_sort", "glx_stage_karras", "glx_stage_aggregate", "glx_sys_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(L.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_python_binding_covers_header():
+    for n in declared_functions():
+        assert getattr(L.lib(), n).argtypes is not None or n == "glx_last_error", n
+
+
+def test_invalid_params_rejected_before_device():
+    lib = L.lib()
+    h = C.c_void_p()
+    for bad in (
+        L.Params(1.0, 0.05, 0.0, 1, 0, 0, 0),        # theta = 0
+        L.Params(1.0, -1.0, 0.5, 1, 0, 0, 0),        # softening < 0
+        L.Params(float("nan"), 0.05, 0.5, 1, 0, 0, 0),
+        L.Params(1.0, 0.05, 0.5, 4, 0, 0, 0),        # leaf_size
+        L.Params(1.0, 0.05, 0.5, 1, 1, 0, 0),        # multipole
+        L.Params(1.0, 0.05, 0.5, 1, 0, 7, 0),        # precision
+    ):
+        assert lib.glx_solver_create(C.byref(bad), C.byref(h)) == L.GLX_EINVAL
+        assert lib.glx_last_error()
+    assert lib.glx_solver_create(None, C.byref(h)) == L.GLX_EINVAL
+
+
+def test_missing_gpu_is_an_error_not_a_fallback():
+    name = C.create_string_buffer(64)
+    if L.lib().glx_device_info(name, 64, None, None) == L.GLX_OK:
+        pytest.skip("a GPU is present")
+    with pytest.raises(L.GlxError) as e:
+        galaxy_amd.Lbvh(1.0, 0.05, 0.5)
+    assert e.value.code == L.GLX_EDEVICE
+
+
+def test_missing_library_raises(monkeypatch, tmp_path):
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(ImportError):
+        L.lib()
