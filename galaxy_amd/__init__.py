@@ -18,4 +18,6 @@ from .lbvh import (  # noqa: F401
     PREC_F32,
     PREC_KAHAN,
     NO_PARENT,
+    WALK_GROUP,
+    WALK_REFERENCE,
 )

@@ -42,8 +42,22 @@ static void validate_params(const glx_params* p) {
   if (!isfinite(p->softening) || p->softening < 0.0) fail(GLX_EINVAL, "softening must be >= 0");
   if (!isfinite(p->theta) || !(p->theta > 0.0)) fail(GLX_EINVAL, "theta must be > 0");
   if (p->leaf_size > 1) fail(GLX_EINVAL, "leaf_size must be 1");
-  if (p->multipole != 0) fail(GLX_EINVAL, "multipole must be 0 (monopole)");
+  if (p->multipole != 0 && p->multipole != 2) fail(GLX_EINVAL, "multipole must be 0 or 2");
   if (p->precision > GLX_PREC_KAHAN) fail(GLX_EINVAL, "unknown precision mode");
+  if (p->morton_bits != 0 && p->morton_bits != 30 && p->morton_bits != 63)
+    fail(GLX_EINVAL, "morton_bits must be 30 or 63");
+  if (p->walk > GLX_WALK_GROUP) fail(GLX_EINVAL, "unknown walk");
+  if (p->multipole == 2 && p->walk != GLX_WALK_GROUP)
+    fail(GLX_EINVAL, "quadrupole moments need the group walk");
+}
+
+static Mode mode_from(const glx_params& p) {
+  Mode m;
+  m.key63 = p.morton_bits == 63;
+  m.group = p.walk == GLX_WALK_GROUP;
+  m.quad = p.multipole == 2;
+  m.kahan = p.precision == GLX_PREC_KAHAN;
+  return m;
 }
 
 static void validate_n(size_t n) {
@@ -90,6 +104,24 @@ void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
   if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "download");
 }
 void sync(hipStream_t st) { check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+
+// Zero the walk statistics (not the bottom-up root counter, which the kernels keep
+// at zero between builds).
+void zero_walk_counters(const SegDesc& d, hipStream_t st) {
+  check(hipMemsetAsync(d.counters + kCntInteractions, 0, 2 * sizeof(unsigned long long), st),
+        "memset counters");
+  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 2 * sizeof(unsigned long long), st),
+        "memset counters");
+}
+
+glx_stats stats_from(const unsigned long long* c) {
+  glx_stats s{};
+  s.point_terms = c[kCntInteractions];
+  s.quad_terms = c[kCntQuad];
+  s.node_tests = c[kCntVisits];
+  s.fallbacks = c[kCntOverflow];
+  return s;
+}
 }  // namespace
 
 extern "C" int glx_device_info(char* name, size_t name_len, int* compute_units,
@@ -117,7 +149,7 @@ struct glx_solver {
   Segment seg;
   Batch* batch = nullptr;
   uint32_t n_cur = 0xFFFFFFFFu;
-  uint64_t stats[2] = {0, 0};
+  glx_stats stats{};
   double* pot_partial = nullptr;
   size_t pot_cap = 0;
   ~glx_solver() {
@@ -138,6 +170,7 @@ extern "C" int glx_solver_create(const glx_params* params, glx_solver** out) {
       s->p = *params;
       s->st = make_stream();
       s->batch = new Batch(s->st);
+      s->batch->set_mode(mode_from(s->p));
     } catch (...) {
       delete s;
       throw;
@@ -171,24 +204,23 @@ extern "C" int glx_solver_accelerations(glx_solver* s, size_t n, const double* p
     validate_n(n);
     if (n == 0) return;
     if (!pos || !mass || !acc) fail(GLX_EINVAL, "NULL array");
-    s->stats[0] = s->stats[1] = 0;
+    s->stats = glx_stats{};
     if (n == 1) {
       acc[0] = acc[1] = acc[2] = 0.0;
       return;
     }
     solver_stage_inputs(s, n, pos, mass);
     const bool count = (s->p.flags & GLX_FLAG_COUNT) != 0;
-    if (count) check(hipMemsetAsync(s->seg.desc.counters, 0, 32, s->st), "memset counters");
+    if (count) zero_walk_counters(s->seg.desc, s->st);
     Batch& b = *s->batch;
     b.reset_bbox();
     b.bbox();
-    b.build_and_walk(0, s->p.precision == GLX_PREC_KAHAN, count);
+    b.build_and_walk(0, count);
     d2h(acc, s->seg.desc.acc_out, 3 * n * sizeof(double), s->st);
-    unsigned long long c[2] = {0, 0};
+    unsigned long long c[kCounters] = {};
     if (count) d2h(c, s->seg.desc.counters, sizeof(c), s->st);
     sync(s->st);
-    s->stats[0] = c[0];
-    s->stats[1] = c[1];
+    s->stats = stats_from(c);
   });
 }
 
@@ -200,11 +232,10 @@ extern "C" int glx_solver_accelerations_active(glx_solver* s, size_t n, const do
   return glx_solver_accelerations(s, n, pos, mass, acc);
 }
 
-extern "C" int glx_solver_stats(glx_solver* s, uint64_t* interactions, uint64_t* node_visits) {
+extern "C" int glx_solver_stats(glx_solver* s, glx_stats* out) {
   return guard([&] {
-    if (!s) fail(GLX_EINVAL, "solver is NULL");
-    if (interactions) *interactions = s->stats[0];
-    if (node_visits) *node_visits = s->stats[1];
+    if (!s || !out) fail(GLX_EINVAL, "NULL argument");
+    *out = s->stats;
   });
 }
 
@@ -266,7 +297,12 @@ struct StageCtx {
   Stream S;
   Segment seg;
   Batch batch;
-  StageCtx() : batch(S.st) {}
+  explicit StageCtx(bool key63 = false, bool quad = false) : batch(S.st) {
+    Mode m;
+    m.key63 = key63;
+    m.quad = quad;
+    batch.set_mode(m);
+  }
   void describe(uint32_t n) {
     seg.reserve(n, false);
     glx_params p{};
@@ -277,77 +313,102 @@ struct StageCtx {
     batch.set_segments({seg.desc});
   }
 };
+
+template <class K>
+void stage_morton(size_t n, const double* pos, double* bounds_out, uint32_t* lanes, K* codes) {
+  validate_n(n);
+  if (n == 0) fail(GLX_EINVAL, "reference_morton requires a non-empty system");
+  if (!pos || !codes) fail(GLX_EINVAL, "NULL array");
+  ensure_device();
+  StageCtx c(sizeof(K) == 8);
+  c.describe((uint32_t)n);
+  DevArray<uint32_t> dl(lanes ? 3 * n : 0);
+  h2d(c.seg.desc.x, pos, 3 * n * sizeof(double), c.S.st);
+  c.batch.reset_bbox();
+  c.batch.bbox();
+  c.batch.encode(dl.p);
+  d2h(codes, c.seg.desc.keys[0], n * sizeof(K), c.S.st);
+  if (lanes) d2h(lanes, dl.p, 3 * n * sizeof(uint32_t), c.S.st);
+  double b[8];
+  d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
+  sync(c.S.st);
+  if (bounds_out) memcpy(bounds_out, b, 5 * sizeof(double));
+}
+
+template <class K>
+void stage_sort(size_t n, const K* codes, uint32_t* order, K* sorted_codes) {
+  validate_n(n);
+  if (n == 0) return;
+  if (!codes) fail(GLX_EINVAL, "NULL array");
+  ensure_device();
+  StageCtx c(sizeof(K) == 8);
+  c.describe((uint32_t)n);
+  h2d(c.seg.desc.keys[0], codes, n * sizeof(K), c.S.st);
+  c.batch.iota();
+  c.batch.sort();
+  if (order) d2h(order, c.seg.desc.vals[0], n * sizeof(uint32_t), c.S.st);
+  if (sorted_codes) d2h(sorted_codes, c.seg.desc.keys[0], n * sizeof(K), c.S.st);
+  sync(c.S.st);
+}
+
+template <class K>
+void stage_karras(size_t n, const K* sorted_codes, uint32_t* left, uint32_t* right,
+                  uint32_t* parent) {
+  validate_n(n);
+  if (n == 0) fail(GLX_EINVAL, "reference_karras requires a non-empty system");
+  if (!sorted_codes || !parent || (n > 1 && (!left || !right))) fail(GLX_EINVAL, "NULL array");
+  for (size_t k = 1; k < n; ++k)
+    if (sorted_codes[k - 1] > sorted_codes[k]) fail(GLX_EINVAL, "codes must be sorted");
+  if (n == 1) {
+    parent[0] = kNoParent;
+    return;
+  }
+  ensure_device();
+  StageCtx c(sizeof(K) == 8);
+  c.describe((uint32_t)n);
+  h2d(c.seg.desc.keys[0], sorted_codes, n * sizeof(K), c.S.st);
+  c.batch.karras();
+  d2h(left, c.seg.desc.left, (n - 1) * sizeof(uint32_t), c.S.st);
+  d2h(right, c.seg.desc.right, (n - 1) * sizeof(uint32_t), c.S.st);
+  d2h(parent, c.seg.desc.parent_leaf, n * sizeof(uint32_t), c.S.st);
+  d2h(parent + n, c.seg.desc.parent_int, (n - 1) * sizeof(uint32_t), c.S.st);
+  sync(c.S.st);
+}
 }  // namespace
 
 extern "C" int glx_stage_morton(size_t n, const double* pos, double* bounds_out, uint32_t* lanes,
                                 uint32_t* codes) {
-  return guard([&] {
-    validate_n(n);
-    if (n == 0) fail(GLX_EINVAL, "reference_morton requires a non-empty system");
-    if (!pos || !codes) fail(GLX_EINVAL, "NULL array");
-    ensure_device();
-    StageCtx c;
-    c.describe((uint32_t)n);
-    DevArray<uint32_t> dl(lanes ? 3 * n : 0);
-    h2d(c.seg.desc.x, pos, 3 * n * sizeof(double), c.S.st);
-    c.batch.reset_bbox();
-    c.batch.bbox();
-    c.batch.encode(dl.p);
-    d2h(codes, c.seg.desc.keys[0], n * sizeof(uint32_t), c.S.st);
-    if (lanes) d2h(lanes, dl.p, 3 * n * sizeof(uint32_t), c.S.st);
-    double b[8];
-    d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
-    sync(c.S.st);
-    if (bounds_out) memcpy(bounds_out, b, 5 * sizeof(double));
-  });
+  return guard([&] { stage_morton<uint32_t>(n, pos, bounds_out, lanes, codes); });
+}
+
+extern "C" int glx_stage_morton63(size_t n, const double* pos, double* bounds_out,
+                                  uint32_t* lanes, uint64_t* codes) {
+  return guard([&] { stage_morton<uint64_t>(n, pos, bounds_out, lanes, codes); });
 }
 
 extern "C" int glx_stage_sort(size_t n, const uint32_t* codes, uint32_t* order,
                               uint32_t* sorted_codes) {
-  return guard([&] {
-    validate_n(n);
-    if (n == 0) return;
-    if (!codes) fail(GLX_EINVAL, "NULL array");
-    ensure_device();
-    StageCtx c;
-    c.describe((uint32_t)n);
-    h2d(c.seg.desc.keys[0], codes, n * sizeof(uint32_t), c.S.st);
-    c.batch.iota();
-    c.batch.sort();
-    if (order) d2h(order, c.seg.desc.vals[0], n * sizeof(uint32_t), c.S.st);
-    if (sorted_codes) d2h(sorted_codes, c.seg.desc.keys[0], n * sizeof(uint32_t), c.S.st);
-    sync(c.S.st);
-  });
+  return guard([&] { stage_sort<uint32_t>(n, codes, order, sorted_codes); });
+}
+
+extern "C" int glx_stage_sort63(size_t n, const uint64_t* codes, uint32_t* order,
+                                uint64_t* sorted_codes) {
+  return guard([&] { stage_sort<uint64_t>(n, codes, order, sorted_codes); });
 }
 
 extern "C" int glx_stage_karras(size_t n, const uint32_t* sorted_codes, uint32_t* left,
                                 uint32_t* right, uint32_t* parent) {
-  return guard([&] {
-    validate_n(n);
-    if (n == 0) fail(GLX_EINVAL, "reference_karras requires a non-empty system");
-    if (!sorted_codes || !parent || (n > 1 && (!left || !right))) fail(GLX_EINVAL, "NULL array");
-    for (size_t k = 1; k < n; ++k)
-      if (sorted_codes[k - 1] > sorted_codes[k]) fail(GLX_EINVAL, "codes must be sorted");
-    if (n == 1) {
-      parent[0] = kNoParent;
-      return;
-    }
-    ensure_device();
-    StageCtx c;
-    c.describe((uint32_t)n);
-    h2d(c.seg.desc.keys[0], sorted_codes, n * sizeof(uint32_t), c.S.st);
-    c.batch.karras();
-    d2h(left, c.seg.desc.left, (n - 1) * sizeof(uint32_t), c.S.st);
-    d2h(right, c.seg.desc.right, (n - 1) * sizeof(uint32_t), c.S.st);
-    d2h(parent, c.seg.desc.parent_leaf, n * sizeof(uint32_t), c.S.st);
-    d2h(parent + n, c.seg.desc.parent_int, (n - 1) * sizeof(uint32_t), c.S.st);
-    sync(c.S.st);
-  });
+  return guard([&] { stage_karras<uint32_t>(n, sorted_codes, left, right, parent); });
+}
+
+extern "C" int glx_stage_karras63(size_t n, const uint64_t* sorted_codes, uint32_t* left,
+                                  uint32_t* right, uint32_t* parent) {
+  return guard([&] { stage_karras<uint64_t>(n, sorted_codes, left, right, parent); });
 }
 
 extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const double* sorted_pos,
                                    const double* sorted_mass, double* origin_out, float* aabb_min,
-                                   float* aabb_max, float* com, float* mass) {
+                                   float* aabb_max, float* com, float* mass, float* quad) {
   return guard([&] {
     validate_n(n);
     if (n == 0) fail(GLX_EINVAL, "reference_aggregate requires a non-empty system");
@@ -356,7 +417,7 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
     for (size_t k = 1; k < n; ++k)
       if (sorted_codes[k - 1] > sorted_codes[k]) fail(GLX_EINVAL, "codes must be sorted");
     ensure_device();
-    StageCtx c;
+    StageCtx c(false, quad != nullptr);
     c.describe((uint32_t)n);
     h2d(c.seg.desc.x, sorted_pos, 3 * n * sizeof(double), c.S.st);
     h2d(c.seg.desc.m, sorted_mass, n * sizeof(double), c.S.st);
@@ -368,11 +429,12 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
     c.batch.gather();
     c.batch.karras();
     c.batch.aggregate();
-    std::vector<float4> lf(n);
+    std::vector<float4> lf(n), qd(quad && n > 1 ? 2 * (n - 1) : 0);
     std::vector<Node> nd(n > 1 ? n - 1 : 0);
     double b[8];
     d2h(lf.data(), c.seg.desc.leaf, n * sizeof(float4), c.S.st);
     if (n > 1) d2h(nd.data(), c.seg.desc.node, (n - 1) * sizeof(Node), c.S.st);
+    if (!qd.empty()) d2h(qd.data(), c.seg.desc.quad, qd.size() * sizeof(float4), c.S.st);
     d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
     sync(c.S.st);
     if (origin_out) memcpy(origin_out, b + 5, 3 * sizeof(double));
@@ -380,6 +442,8 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
       const float p[3] = {lf[k].x, lf[k].y, lf[k].z};
       for (int a = 0; a < 3; ++a) aabb_min[3 * k + a] = aabb_max[3 * k + a] = com[3 * k + a] = p[a];
       mass[k] = lf[k].w;
+      if (quad)
+        for (int w = 0; w < 6; ++w) quad[6 * k + w] = 0.f;
     }
     for (size_t j = 0; j + 1 < n; ++j) {
       const size_t u = n + j;
@@ -392,11 +456,16 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
         aabb_max[3 * u + a] = mx[a];
       }
       mass[u] = q.a.w;
+      if (quad) {
+        const float qq[6] = {qd[2 * j].x, qd[2 * j].y, qd[2 * j].z,
+                             qd[2 * j].w, qd[2 * j + 1].x, qd[2 * j + 1].y};
+        for (int w = 0; w < 6; ++w) quad[6 * u + w] = qq[w];
+      }
     }
   });
 }
 
-// ----------------------------------------------------------------------------------
+
 // resident stepper
 // ----------------------------------------------------------------------------------
 
@@ -434,6 +503,7 @@ extern "C" int glx_sys_create(const glx_params* params, glx_sys** out) {
       s->p = *params;
       s->st = make_stream();
       s->batch = new Batch(s->st);
+      s->batch->set_mode(mode_from(s->p));
     } catch (...) {
       delete s;
       throw;
@@ -470,7 +540,7 @@ extern "C" int glx_sys_upload(glx_sys* s, size_t n, const double* pos, const dou
     Batch& b = *s->batch;
     b.reset_bbox();
     b.bbox();
-    b.build_and_walk(0, s->p.precision == GLX_PREC_KAHAN, false);  // prime a(x0)
+    b.build_and_walk(0, false);  // prime a(x0)
     sync(s->st);
     s->n = (uint32_t)n;
   });
@@ -486,17 +556,14 @@ extern "C" int glx_sys_step(glx_sys* s, double dt, uint64_t k) {
     }
     if (k == 0) return;
     if (!(dt == s->cur_dt)) {
-      std::vector<SegDesc> d = s->batch->host_descs();
-      d[0].dt = dt;
-      d[0].half_dt = (float)(0.5 * dt);
-      s->batch->set_segments(d);
+      s->batch->set_dt(dt);
       s->cur_dt = dt;
     }
     if (!s->exec) {
       check(hipStreamBeginCapture(s->st, hipStreamCaptureModeRelaxed), "BeginCapture");
       try {
         s->batch->kick_drift_bbox();
-        s->batch->build_and_walk(1, s->p.precision == GLX_PREC_KAHAN, false);
+        s->batch->build_and_walk(1, false);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s->st, &g);
@@ -511,6 +578,81 @@ extern "C" int glx_sys_step(glx_sys* s, double dt, uint64_t k) {
       s->time += dt;
     }
     sync(s->st);
+  });
+}
+
+// Phases of one resident KDK step, in launch order.
+enum { kPhKick = 0, kPhEncode, kPhSort, kPhGather, kPhKarras, kPhAggregate, kPhWalk, kPhases };
+
+extern "C" int glx_sys_step_timed(glx_sys* s, double dt, uint64_t k, double* phase_ms) {
+  return guard([&] {
+    if (!s || !phase_ms) fail(GLX_EINVAL, "NULL argument");
+    if (!isfinite(dt)) fail(GLX_EINVAL, "dt must be finite");
+    for (int q = 0; q < kPhases; ++q) phase_ms[q] = 0.0;
+    if (s->n == 0 || k == 0) {
+      s->time += (double)k * dt;
+      return;
+    }
+    if (!(dt == s->cur_dt)) {
+      s->batch->set_dt(dt);
+      s->cur_dt = dt;
+    }
+    const size_t ne = (size_t)k * (kPhases + 1);
+    std::vector<hipEvent_t> ev(ne, nullptr);
+    struct EvGuard {
+      std::vector<hipEvent_t>& e;
+      ~EvGuard() {
+        for (auto x : e)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } eg{ev};
+    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    Batch& b = *s->batch;
+    for (uint64_t q = 0; q < k; ++q) {
+      hipEvent_t* e = &ev[q * (kPhases + 1)];
+      check(hipEventRecord(e[0], s->st), "hipEventRecord");
+      b.kick_drift_bbox();
+      check(hipEventRecord(e[1], s->st), "hipEventRecord");
+      b.encode();
+      check(hipEventRecord(e[2], s->st), "hipEventRecord");
+      b.sort();
+      check(hipEventRecord(e[3], s->st), "hipEventRecord");
+      b.gather();
+      check(hipEventRecord(e[4], s->st), "hipEventRecord");
+      b.karras();
+      check(hipEventRecord(e[5], s->st), "hipEventRecord");
+      b.aggregate();
+      check(hipEventRecord(e[6], s->st), "hipEventRecord");
+      b.walk(1, false);
+      check(hipEventRecord(e[7], s->st), "hipEventRecord");
+      s->time += dt;
+    }
+    sync(s->st);
+    for (uint64_t q = 0; q < k; ++q) {
+      hipEvent_t* e = &ev[q * (kPhases + 1)];
+      for (int ph = 0; ph < kPhases; ++ph) {
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, e[ph], e[ph + 1]), "hipEventElapsedTime");
+        phase_ms[ph] += ms;
+      }
+    }
+  });
+}
+
+extern "C" int glx_sys_count(glx_sys* s, glx_stats* out) {
+  return guard([&] {
+    if (!s || !out) fail(GLX_EINVAL, "NULL argument");
+    unsigned long long c[kCounters] = {};
+    if (s->n > 1) {
+      Batch& b = *s->batch;
+      zero_walk_counters(s->seg.desc, s->st);
+      b.reset_bbox();
+      b.bbox();
+      b.build_and_walk(0, true);
+      d2h(c, s->seg.desc.counters, sizeof(c), s->st);
+      sync(s->st);
+    }
+    *out = stats_from(c);
   });
 }
 

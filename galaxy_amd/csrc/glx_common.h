@@ -7,7 +7,7 @@
 //
 // Per-segment index spaces (all local to the segment):
 //   particles        i in [0, n)          original order (the caller's index order)
-//   sorted leaves    k in [0, n)          Morton order; leaf k holds particle vals[k]
+//   sorted leaves    k in [0, n)          Morton order; leaf k holds particle vals[0][k]
 //   internal nodes   j in [0, n-1)        Karras numbering; unified index n + j
 //   unified index    u < n  -> leaf u,    u >= n -> internal node u - n, root = n
 #pragma once
@@ -18,25 +18,42 @@ namespace glx {
 
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;  // solvers/src/lbvh.rs:244
 constexpr uint32_t kEnd = 0xFFFFFFFFu;       // walk terminator
-constexpr uint32_t kMortonBits = 10;         // solvers/src/lbvh.rs:32
 constexpr int kRadixBits = 8;
 constexpr int kRadixBins = 1 << kRadixBits;
-constexpr int kRadixPasses = 4;              // 30-bit codes, 8-bit digits
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
-constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
-constexpr int kBlock = 256;                   // threads per block of element kernels
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per element tile
+constexpr int kBlock = 256;                            // threads per block of element kernels
+constexpr int kAggTile = 1024;                         // leaves per bottom-up tile (LDS pass)
+constexpr int kGroup = 64;                             // targets per group walk (one wave)
 
-// Walk record of one internal node (64 B, one aligned half cache line).
+// Walk record of one internal node (64 B).
 struct __align__(16) Node {
   float4 a;  // centre of mass (origin-relative f32), mass
-  float4 b;  // AABB min (origin-relative f32), squared acceptance radius
-  float4 c;  // AABB max (origin-relative f32), left child (unified index, bits)
-  uint4 d;   // next node after this subtree (unified or kEnd), spare x3
+  float4 b;  // AABB min (origin-relative f32), squared acceptance radius (s/theta + delta)^2
+  float4 c;  // AABB max (origin-relative f32), left child (unified index, as bits)
+  uint4 d;   // next node after this subtree (unified or kEnd), right child, first leaf, last leaf
 };
 
-// Precision of the walk's accumulation.
+// Build/walk configuration shared by all segments of a launch sequence.
+struct Mode {
+  bool key63 = false;    // 63-bit Morton keys (2^21 cells per axis) instead of 30-bit
+  bool group = false;    // wave-cooperative group walk instead of the per-target walk
+  bool quad = false;     // quadrupole moments (group walk only)
+  bool kahan = false;    // Kahan-compensated accumulation
+};
+
 enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1 };
+
+// counters[] slots
+enum : int {
+  kCntInteractions = 0,  // point-mass terms (leaf bodies and monopole nodes), per target
+  kCntVisits = 1,        // node tests
+  kCntRoots = 2,         // local subtree roots handed to the top bottom-up pass
+  kCntOverflow = 3,      // group walks that fell back to the per-target walk
+  kCntQuad = 4,          // quadrupole-node terms, per target
+  kCounters = 8
+};
 
 struct SegDesc {
   // resident state (original particle order)
@@ -46,23 +63,25 @@ struct SegDesc {
   float* acc;       // 3n accelerations (f32, already x G) cached between steps
   double* acc_out;  // optional 3n f64 output of a one-shot force evaluation
   // build scratch
-  uint32_t* keys[2];
-  uint32_t* vals[2];
+  void* keys[2];         // sort keys: uint32_t (30-bit) or uint64_t (63-bit)
+  uint32_t* vals[2];     // sort payload (original index)
   float4* leaf;          // n sorted leaves: origin-relative f32 position, mass
   Node* node;            // n-1 internal-node walk records
+  float4* quad;          // 2(n-1) quadrupole moments {xx,yy,zz,xy},{xz,yz,-,-} or null
   uint32_t* left;        // n-1 left child (unified)
   uint32_t* right;       // n-1 right child (unified)
   uint32_t* parent_leaf; // n parent of leaf k (unified)
   uint32_t* parent_int;  // n-1 parent of internal node j (unified)
   uint32_t* rs;          // n-1 right child of the internal node whose split is s
+  uint32_t* lo;          // n-1 first leaf of internal node j's range
   uint32_t* hi;          // n-1 last leaf of internal node j's range
-  uint32_t* flag;        // n-1 arrival counters of the bottom-up pass (kept at 0)
-  unsigned long long* bbox;  // 6 order-preserving encodings: min xyz, max xyz
-  double* bounds;        // 8: bmin xyz, size, scale, origin xyz
-  unsigned long long* counters;  // 4 walk statistics (interactions, node visits)
+  uint32_t* flag;        // n-1 arrival counters of the top bottom-up pass (kept at 0)
+  uint32_t* roots;       // n local subtree roots handed from the tile pass to the top pass
+  unsigned long long* bbox;      // 6 order-preserving encodings: min xyz, max xyz
+  double* bounds;                // 8: bmin xyz, size, scale, origin xyz
+  unsigned long long* counters;  // kCounters: see kCnt*
   uint32_t n;
-  uint32_t sort_tile0;   // first sort tile of this segment in the tile tables
-  uint32_t sort_tiles;   // number of sort tiles
+  uint32_t sort_tile0, sort_tiles;  // element tiles (kSortTile) of this segment
   uint32_t precision;
   float g, eps2, theta, half_dt;
   double dt;

@@ -10,17 +10,19 @@ hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg);
 hipError_t launch_bbox(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
 hipError_t launch_kick_drift_bbox(hipStream_t st, const SegDesc* segs, const Tile* et,
                                   uint32_t net);
-hipError_t launch_encode(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net,
-                         uint32_t* lanes_out);
-hipError_t launch_bounds_only(hipStream_t st, const SegDesc* segs, uint32_t nseg);
+hipError_t launch_encode(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
+                         uint32_t net, uint32_t* lanes_out);
+hipError_t launch_bounds_only(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg);
 hipError_t launch_iota(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
-hipError_t launch_sort(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
-                       uint32_t net, uint32_t* tile_hist, uint32_t* tile_off);
+hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
+                       const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off);
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
-hipError_t launch_karras(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
-hipError_t launch_aggregate(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
-hipError_t launch_walk(hipStream_t st, const SegDesc* segs, const Tile* wt, uint32_t nwt,
-                       int mode, bool kahan, bool count);
+hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
+                         uint32_t net);
+hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
+                            const Tile* at, uint32_t nat);
+hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* wt,
+                       uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count);
 hipError_t launch_diag(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
                        uint32_t net, double* partial, double* out);
 hipError_t launch_direct_f64(hipStream_t st, const double* x, const double* m, uint32_t n,

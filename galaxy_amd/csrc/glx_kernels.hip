@@ -32,6 +32,14 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
 }
+__device__ __forceinline__ float wave_minf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -42,8 +50,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // Block-wide exclusive scan of one u32 per thread (blockDim == kBlock == 256).
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh_wave,
-                                                         uint32_t* total) {
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh_wave) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t inc = v;
   for (int o = 1; o < 64; o <<= 1) {
@@ -52,14 +59,9 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   }
   if (lane == 63) sh_wave[w] = inc;
   __syncthreads();
-  uint32_t pre = 0, tot = 0;
-  for (int k = 0; k < kBlock / 64; ++k) {
-    uint32_t c = sh_wave[k];
-    if (k < w) pre += c;
-    tot += c;
-  }
+  uint32_t pre = 0;
+  for (int k = 0; k < w; ++k) pre += sh_wave[k];
   __syncthreads();
-  if (total) *total = tot;
   return pre + inc - v;
 }
 
@@ -108,16 +110,46 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   x = (x | (x << 2)) & 0x09249249u;
   return x;
 }
-__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-  return expand10(x) | (expand10(y) << 1) | (expand10(z) << 2);
+// the same spread over 21 bits (63-bit codes)
+__device__ __forceinline__ uint64_t expand21(uint64_t x) {
+  x &= 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
 }
+
+template <class K>
+struct KeyTraits;
+template <>
+struct KeyTraits<uint32_t> {
+  static constexpr int cell_bits = 10;  // solvers/src/lbvh.rs:32
+  static constexpr int passes = 4;
+  static constexpr int tie_base = 32;
+  __device__ static uint32_t code(uint32_t x, uint32_t y, uint32_t z) {
+    return expand10(x) | (expand10(y) << 1) | (expand10(z) << 2);
+  }
+  __device__ static int clz(uint32_t v) { return __clz((int)v); }
+};
+template <>
+struct KeyTraits<uint64_t> {
+  static constexpr int cell_bits = 21;  // the 63-bit refinement of solvers/src/lbvh.rs:29-31
+  static constexpr int passes = 8;
+  static constexpr int tie_base = 64;
+  __device__ static uint64_t code(uint32_t x, uint32_t y, uint32_t z) {
+    return expand21(x) | (expand21(y) << 1) | (expand21(z) << 2);
+  }
+  __device__ static int clz(uint64_t v) { return __clzll((long long)v); }
+};
 
 // solvers/src/lbvh.rs:110-120: the padded root cube of the segment's bbox, in f64
 // with the reference's operation order (no contraction). origin = cube centre.
 struct Bounds {
   double bmin[3], size, scale, origin[3];
 };
-__device__ Bounds seg_bounds(const unsigned long long* bbox) {
+__device__ Bounds seg_bounds(const unsigned long long* bbox, int cell_bits) {
 #pragma clang fp contract(off)
   Bounds b;
   double lo[3], hi[3], ext[3];
@@ -133,14 +165,14 @@ __device__ Bounds seg_bounds(const unsigned long long* bbox) {
   const double half = fmax(0.5 * maxe, 1e-12) * (1.0 + 1e-9);
   for (int a = 0; a < 3; ++a) b.bmin[a] = b.origin[a] - half;
   b.size = 2.0 * half;
-  b.scale = (double)(1u << kMortonBits) / b.size;
+  b.scale = (double)(1u << cell_bits) / b.size;
   return b;
 }
 
-// solvers/src/lbvh.rs:56-64: floor, clamp below at 0 (NaN -> 0), saturate at 1023.
-__device__ __forceinline__ uint32_t quant_lane(double v) {
+// solvers/src/lbvh.rs:56-64: floor, clamp below at 0 (NaN -> 0), saturate at the top cell.
+__device__ __forceinline__ uint32_t quant_lane(double v, double top) {
   const double f = fmax(floor(v), 0.0);
-  return f >= 1023.0 ? 1023u : (uint32_t)f;
+  return f >= top ? (uint32_t)top : (uint32_t)f;
 }
 
 // ----------------------------------------------------------------------------------
@@ -195,17 +227,21 @@ k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
 }
 
 // ----------------------------------------------------------------------------------
-// 2. Morton encode (bit-exact with solvers/src/lbvh.rs:105-129)
+// 2. Morton encode (bit-exact with solvers/src/lbvh.rs:105-129; 63-bit variant with
+//    2^21 cells per axis)
 // ----------------------------------------------------------------------------------
 
+template <class K>
 __global__ void __launch_bounds__(kBlock)
 k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
          uint32_t* __restrict__ lanes_out) {
 #pragma clang fp contract(off)
+  using T = KeyTraits<K>;
+  const double top = (double)((1u << T::cell_bits) - 1u);
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    const Bounds b = seg_bounds(s.bbox);
+    const Bounds b = seg_bounds(s.bbox, T::cell_bits);
     if (tl.start == 0 && threadIdx.x == 0) {
       s.bounds[0] = b.bmin[0];
       s.bounds[1] = b.bmin[1];
@@ -216,27 +252,29 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       s.bounds[6] = b.origin[1];
       s.bounds[7] = b.origin[2];
     }
+    K* keys = static_cast<K*>(s.keys[0]);
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const uint32_t i = tl.start + e;
       uint32_t l[3];
-      for (int a = 0; a < 3; ++a) l[a] = quant_lane((s.x[3 * (size_t)i + a] - b.bmin[a]) * b.scale);
+      for (int a = 0; a < 3; ++a)
+        l[a] = quant_lane((s.x[3 * (size_t)i + a] - b.bmin[a]) * b.scale, top);
       if (lanes_out) {
         lanes_out[3 * (size_t)i] = l[0];
         lanes_out[3 * (size_t)i + 1] = l[1];
         lanes_out[3 * (size_t)i + 2] = l[2];
       }
-      s.keys[0][i] = morton3(l[0], l[1], l[2]);
+      keys[i] = T::code(l[0], l[1], l[2]);
       s.vals[0][i] = i;
     }
   }
 }
 
 // Bounds only (used when keys are supplied by the caller).
-__global__ void k_bounds(const SegDesc* __restrict__ segs, uint32_t nseg) {
+__global__ void k_bounds(const SegDesc* __restrict__ segs, uint32_t nseg, int cell_bits) {
   const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (sidx >= nseg) return;
   const SegDesc& s = segs[sidx];
-  const Bounds b = seg_bounds(s.bbox);
+  const Bounds b = seg_bounds(s.bbox, cell_bits);
   for (int a = 0; a < 3; ++a) {
     s.bounds[a] = b.bmin[a];
     s.bounds[5 + a] = b.origin[a];
@@ -261,6 +299,7 @@ k_iota(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_
 //    is stable.
 // ----------------------------------------------------------------------------------
 
+template <class K>
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_upsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
                 uint32_t* __restrict__ tile_hist, int shift, int src) {
@@ -268,38 +307,64 @@ k_radix_upsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    const uint32_t* keys = s.keys[src] + tl.start;
+    const K* keys = static_cast<const K*>(s.keys[src]) + tl.start;
     hist[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < tl.count; e += kSortThreads)
-      atomicAdd(&hist[(keys[e] >> shift) & (kRadixBins - 1)], 1u);
+      atomicAdd(&hist[(uint32_t)(keys[e] >> shift) & (kRadixBins - 1)], 1u);
     __syncthreads();
     tile_hist[(size_t)t * kRadixBins + threadIdx.x] = hist[threadIdx.x];
     __syncthreads();
   }
 }
 
-// One block per segment: per digit, exclusive scan over the segment's tiles plus the
-// segment-wide digit base (exclusive scan over digits of the digit totals).
-__global__ void __launch_bounds__(kSortThreads)
+// One block of 1024 threads per segment: thread (digit d, chunk g) scans a quarter of
+// the segment's tiles; tile offsets = digit base (exclusive scan over digits of the
+// digit totals) + exclusive prefix over earlier tiles.
+constexpr int kScanThreads = 1024;
+constexpr int kScanChunks = kScanThreads / kRadixBins;
+
+__global__ void __launch_bounds__(kScanThreads)
 k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ tile_hist,
              uint32_t* __restrict__ tile_off) {
-  __shared__ uint32_t shw[kSortThreads / 64];
-  const uint32_t d = threadIdx.x;
+  __shared__ uint32_t part[kScanChunks][kRadixBins];
+  __shared__ uint32_t base[kRadixBins];
+  __shared__ uint32_t wsum[kRadixBins / 64];
+  const uint32_t d = threadIdx.x & (kRadixBins - 1), g = threadIdx.x / kRadixBins;
   for (uint32_t sidx = blockIdx.x; sidx < nseg; sidx += gridDim.x) {
     const SegDesc& s = segs[sidx];
-    const uint32_t t0 = s.sort_tile0, t1 = s.sort_tile0 + s.sort_tiles;
-    uint32_t tot = 0;
-    for (uint32_t t = t0; t < t1; ++t) tot += tile_hist[(size_t)t * kRadixBins + d];
-    uint32_t run = block_exclusive_scan(tot, shw, nullptr);
-    for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t nt = s.sort_tiles, per = (nt + kScanChunks - 1) / kScanChunks;
+    const uint32_t c0 = s.sort_tile0 + min(nt, g * per), c1 = s.sort_tile0 + min(nt, (g + 1) * per);
+    uint32_t sum = 0;
+    for (uint32_t t = c0; t < c1; ++t) sum += tile_hist[(size_t)t * kRadixBins + d];
+    part[g][d] = sum;
+    __syncthreads();
+    if (threadIdx.x < kRadixBins) {
+      uint32_t tot = 0;
+      for (int q = 0; q < kScanChunks; ++q) tot += part[q][d];
+      const int lane = d & 63, w = d >> 6;
+      uint32_t inc = tot;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(inc, o);
+        if (lane >= o) inc += v;
+      }
+      if (lane == 63) wsum[w] = inc;
+      base[d] = inc - tot;
+    }
+    __syncthreads();
+    uint32_t run = base[d];
+    for (uint32_t w = 0; w < (d >> 6); ++w) run += wsum[w];
+    for (uint32_t q = 0; q < g; ++q) run += part[q][d];
+    for (uint32_t t = c0; t < c1; ++t) {
       const uint32_t h = tile_hist[(size_t)t * kRadixBins + d];
       tile_off[(size_t)t * kRadixBins + d] = run;
       run += h;
     }
+    __syncthreads();
   }
 }
 
+template <class K>
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
                   const uint32_t* __restrict__ tile_off, int shift, int src) {
@@ -307,7 +372,7 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
   constexpr int PER_WAVE = kSortTile / W;
   __shared__ uint32_t wcnt[W][kRadixBins];
   __shared__ uint32_t dstart[kRadixBins];
-  __shared__ uint32_t skey[kSortTile];
+  __shared__ K skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   __shared__ uint32_t shw[W];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -315,21 +380,23 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    const uint32_t* keys_in = s.keys[src] + tl.start;
+    const K* keys_in = static_cast<const K*>(s.keys[src]) + tl.start;
     const uint32_t* vals_in = s.vals[src] + tl.start;
-    uint32_t* keys_out = s.keys[src ^ 1];
+    K* keys_out = static_cast<K*>(s.keys[src ^ 1]);
     uint32_t* vals_out = s.vals[src ^ 1];
     for (int k = 0; k < W; ++k) wcnt[k][threadIdx.x] = 0;
     __syncthreads();
 
-    uint32_t key[kSortItems], val[kSortItems], dig[kSortItems], rank[kSortItems];
+    K key[kSortItems];
+    uint32_t val[kSortItems], dig[kSortItems], rank[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
       const uint32_t e = w * PER_WAVE + j * 64 + lane;  // striped: index order = (j, lane)
       const bool valid = e < tl.count;
-      key[j] = valid ? keys_in[e] : 0u;
+      key[j] = valid ? keys_in[e] : (K)0;
       val[j] = valid ? vals_in[e] : 0u;
-      const uint32_t dd = valid ? ((key[j] >> shift) & (kRadixBins - 1)) : (uint32_t)kRadixBins;
+      const uint32_t dd =
+          valid ? ((uint32_t)(key[j] >> shift) & (kRadixBins - 1)) : (uint32_t)kRadixBins;
       // lanes holding the same digit (bit kRadixBits separates the invalid tail)
       unsigned long long peers = ~0ull;
 #pragma unroll
@@ -360,7 +427,7 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
         wcnt[k][d] = run;
         run += c[k];
       }
-      dstart[d] = block_exclusive_scan(tot, shw, nullptr);
+      dstart[d] = block_exclusive_scan(tot, shw);
     }
     __syncthreads();
 #pragma unroll
@@ -374,8 +441,8 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
     __syncthreads();
     const uint32_t* off = tile_off + (size_t)t * kRadixBins;
     for (uint32_t p = threadIdx.x; p < tl.count; p += kSortThreads) {
-      const uint32_t k = skey[p];
-      const uint32_t d = (k >> shift) & (kRadixBins - 1);
+      const K k = skey[p];
+      const uint32_t d = (uint32_t)(k >> shift) & (kRadixBins - 1);
       const uint32_t dst = off[d] + (p - dstart[d]);
       keys_out[dst] = k;
       vals_out[dst] = sval[p];
@@ -411,24 +478,26 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 
 // ----------------------------------------------------------------------------------
 // 5. Karras radix-tree build (solvers/src/lbvh.rs:153-240,276-306), signed int
-//    indices; each internal node also records its range end and publishes itself
+//    indices; each internal node also records its leaf range and publishes itself
 //    as the right child of its split position, from which the skip pointers follow.
 // ----------------------------------------------------------------------------------
 
-__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n, int a, int b) {
+template <class K>
+__device__ __forceinline__ int kdelta(const K* __restrict__ codes, int n, int a, int b) {
   if (b < 0 || b >= n) return -1;
-  const uint32_t ca = codes[a], cb = codes[b];
-  if (ca == cb) return 32 + __clz((int)((uint32_t)a ^ (uint32_t)b));
-  return __clz((int)(ca ^ cb));
+  const K ca = codes[a], cb = codes[b];
+  if (ca == cb) return KeyTraits<K>::tie_base + __clz((int)((uint32_t)a ^ (uint32_t)b));
+  return KeyTraits<K>::clz(ca ^ cb);
 }
 
+template <class K>
 __global__ void __launch_bounds__(kBlock)
 k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const int n = (int)s.n;
-    const uint32_t* codes = s.keys[0];
+    const K* codes = static_cast<const K*>(s.keys[0]);
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const int i = (int)(tl.start + e);
       if (i >= n - 1) continue;
@@ -457,6 +526,7 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       if (lu < (uint32_t)n) s.parent_leaf[lu] = me; else s.parent_int[lu - n] = me;
       if (ru < (uint32_t)n) s.parent_leaf[ru] = me; else s.parent_int[ru - n] = me;
       s.rs[gamma] = ru;
+      s.lo[i] = (uint32_t)lo;
       s.hi[i] = (uint32_t)hi;
       if (i == 0) s.parent_int[0] = kNoParent;
     }
@@ -464,102 +534,219 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 }
 
 // ----------------------------------------------------------------------------------
-// 6. bottom-up aggregate with arrival flags (fold order fixed: left, right;
-//    solvers/src/lbvh.rs:623-636). One thread per leaf climbs while it is the second
-//    arrival at a parent; the flag is reset by that second arrival so the array is
-//    all-zero again for the next build.
+// 6. bottom-up aggregate (fold order fixed: left, right; solvers/src/lbvh.rs:623-636),
+//    in two passes that need only workgroup-scope ordering:
+//    (a) per tile of kAggTile sorted leaves, one block climbs from every leaf with
+//        arrival counters in LDS and keeps in-tile node aggregates in LDS; a climb
+//        stops below the first node whose leaf range leaves the tile and hands that
+//        subtree root to (b);
+//    (b) one block per segment climbs from the handed-over roots through the nodes
+//        that straddle tiles, with arrival counters in global memory.
+//    Each completed node is written once as its walk record (and quadrupole).
 // ----------------------------------------------------------------------------------
 
-struct Agg {
+struct AggV {
   float3 com, mn, mx;
   float mass;
+  float q[6];  // traceless quadrupole xx, yy, zz, xy, xz, yz about com
 };
 
-__device__ __forceinline__ Agg load_child(const SegDesc& s, uint32_t c, uint32_t n) {
-  Agg r;
-  if (c < n) {
-    const float4 p = s.leaf[c];
-    r.com = make_float3(p.x, p.y, p.z);
-    r.mn = r.com;
-    r.mx = r.com;
-    r.mass = p.w;
+__device__ __forceinline__ AggV leaf_agg(float4 p) {
+  AggV r;
+  r.com = make_float3(p.x, p.y, p.z);
+  r.mn = r.com;
+  r.mx = r.com;
+  r.mass = p.w;
+  for (int c = 0; c < 6; ++c) r.q[c] = 0.f;
+  return r;
+}
+
+template <bool QUAD>
+__device__ __forceinline__ AggV fold_agg(const AggV& l, const AggV& r) {
+  AggV o;
+  o.mass = l.mass + r.mass;
+  if (o.mass > 0.0f) {
+    o.com.x = (l.com.x * l.mass + r.com.x * r.mass) / o.mass;
+    o.com.y = (l.com.y * l.mass + r.com.y * r.mass) / o.mass;
+    o.com.z = (l.com.z * l.mass + r.com.z * r.mass) / o.mass;
   } else {
-    const Node* nd = &s.node[c - n];
-    const float4 a = nd->a, b = nd->b, cc = nd->c;
-    r.com = make_float3(a.x, a.y, a.z);
-    r.mass = a.w;
-    r.mn = make_float3(b.x, b.y, b.z);
-    r.mx = make_float3(cc.x, cc.y, cc.z);
+    o.com.x = (l.mn.x + l.mx.x + r.mn.x + r.mx.x) * 0.25f;
+    o.com.y = (l.mn.y + l.mx.y + r.mn.y + r.mx.y) * 0.25f;
+    o.com.z = (l.mn.z + l.mx.z + r.mn.z + r.mx.z) * 0.25f;
+  }
+  o.mn = make_float3(fminf(l.mn.x, r.mn.x), fminf(l.mn.y, r.mn.y), fminf(l.mn.z, r.mn.z));
+  o.mx = make_float3(fmaxf(l.mx.x, r.mx.x), fmaxf(l.mx.y, r.mx.y), fmaxf(l.mx.z, r.mx.z));
+  for (int c = 0; c < 6; ++c) o.q[c] = 0.f;
+  if (QUAD && o.mass > 0.0f) {
+    // parallel-axis shift of each child's traceless quadrupole to the parent com
+    const AggV* ch[2] = {&l, &r};
+    for (int k = 0; k < 2; ++k) {
+      const AggV& c = *ch[k];
+      const float dx = c.com.x - o.com.x, dy = c.com.y - o.com.y, dz = c.com.z - o.com.z;
+      const float d2 = dx * dx + dy * dy + dz * dz, m = c.mass;
+      o.q[0] += c.q[0] + m * (3.f * dx * dx - d2);
+      o.q[1] += c.q[1] + m * (3.f * dy * dy - d2);
+      o.q[2] += c.q[2] + m * (3.f * dz * dz - d2);
+      o.q[3] += c.q[3] + m * 3.f * dx * dy;
+      o.q[4] += c.q[4] + m * 3.f * dx * dz;
+      o.q[5] += c.q[5] + m * 3.f * dy * dz;
+    }
+  }
+  return o;
+}
+
+// Walk record of internal node j from its aggregate (solvers/src/lbvh.rs:602-611 for
+// centre/half/delta; the acceptance radius follows from :511-514:
+// theta*(d - delta) >= s  <=>  d >= s/theta + delta).
+template <bool QUAD>
+__device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const AggV& r,
+                                           float inv_theta) {
+  const uint32_t n = s.n;
+  const float cx = (r.mn.x + r.mx.x) * 0.5f, cy = (r.mn.y + r.mx.y) * 0.5f,
+              cz = (r.mn.z + r.mx.z) * 0.5f;
+  const float hx = (r.mx.x - r.mn.x) * 0.5f, hy = (r.mx.y - r.mn.y) * 0.5f,
+              hz = (r.mx.z - r.mn.z) * 0.5f;
+  const float size = 2.0f * fmaxf(hx, fmaxf(hy, hz));
+  const float ddx = r.com.x - cx, ddy = r.com.y - cy, ddz = r.com.z - cz;
+  const float delta = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+  const float rcrit = size * inv_theta + delta;
+  const uint32_t h = s.hi[j];
+  Node nd;
+  nd.a = make_float4(r.com.x, r.com.y, r.com.z, r.mass);
+  nd.b = make_float4(r.mn.x, r.mn.y, r.mn.z, rcrit * rcrit);
+  nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(s.left[j]));
+  nd.d = make_uint4(h == n - 1 ? kEnd : s.rs[h], s.right[j], s.lo[j], h);
+  s.node[j] = nd;
+  if (QUAD) {
+    s.quad[2 * j] = make_float4(r.q[0], r.q[1], r.q[2], r.q[3]);
+    s.quad[2 * j + 1] = make_float4(r.q[4], r.q[5], 0.f, 0.f);
+  }
+}
+
+template <bool QUAD>
+__device__ __forceinline__ AggV global_child(const SegDesc& s, uint32_t c) {
+  const uint32_t n = s.n;
+  if (c < n) return leaf_agg(s.leaf[c]);
+  const Node& nd = s.node[c - n];
+  AggV r;
+  const float4 a = nd.a, b = nd.b, cc = nd.c;
+  r.com = make_float3(a.x, a.y, a.z);
+  r.mass = a.w;
+  r.mn = make_float3(b.x, b.y, b.z);
+  r.mx = make_float3(cc.x, cc.y, cc.z);
+  if (QUAD) {
+    const float4 q0 = s.quad[2 * (c - n)], q1 = s.quad[2 * (c - n) + 1];
+    r.q[0] = q0.x; r.q[1] = q0.y; r.q[2] = q0.z; r.q[3] = q0.w; r.q[4] = q1.x; r.q[5] = q1.y;
+  } else {
+    for (int k = 0; k < 6; ++k) r.q[k] = 0.f;
   }
   return r;
 }
 
+template <bool QUAD>
 __global__ void __launch_bounds__(kBlock)
-k_aggregate(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
+k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
+                 uint32_t ntiles) {
+  constexpr int NQ = QUAD ? 6 : 1;
+  __shared__ float s_com[3][kAggTile], s_mass[kAggTile], s_mn[3][kAggTile], s_mx[3][kAggTile];
+  __shared__ float s_q[NQ][kAggTile];
+  __shared__ uint32_t s_flag[kAggTile];
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const uint32_t n = s.n;
     if (n < 2) continue;
+    const uint32_t t0 = tl.start, t1 = tl.start + tl.count;
     const float inv_theta = 1.0f / s.theta;
+    for (uint32_t q = threadIdx.x; q < kAggTile; q += kBlock) s_flag[q] = 0u;
+    __syncthreads();
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
-      const uint32_t k = tl.start + e;
+      const uint32_t k = t0 + e;
+      uint32_t u = k;
       uint32_t p = s.parent_leaf[k];
-      // a climb visits each ancestor once; the bound only guards corrupted input
+      // each ancestor is visited once; the bound only guards corrupted input
       for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {
         const uint32_t j = p - n;
         if (j >= n - 1) break;
-        __threadfence();
-        const uint32_t old =
-            __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) break;  // first arrival: the sibling's thread finishes this node
-        __threadfence();
-        __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t lc = s.left[j], rc = s.right[j];
-        const Agg la = load_child(s, lc, n), ra = load_child(s, rc, n);
-        Agg r;
-        r.mass = la.mass + ra.mass;
-        if (r.mass > 0.0f) {
-          r.com.x = (la.com.x * la.mass + ra.com.x * ra.mass) / r.mass;
-          r.com.y = (la.com.y * la.mass + ra.com.y * ra.mass) / r.mass;
-          r.com.z = (la.com.z * la.mass + ra.com.z * ra.mass) / r.mass;
-        } else {
-          r.com.x = (la.mn.x + la.mx.x + ra.mn.x + ra.mx.x) * 0.25f;
-          r.com.y = (la.mn.y + la.mx.y + ra.mn.y + ra.mx.y) * 0.25f;
-          r.com.z = (la.mn.z + la.mx.z + ra.mn.z + ra.mx.z) * 0.25f;
+        if (s.lo[j] < t0 || s.hi[j] >= t1) {
+          const unsigned long long slot = atomicAdd(&s.counters[kCntRoots], 1ull);
+          if (slot < n) s.roots[slot] = u;
+          break;
         }
-        r.mn = make_float3(fminf(la.mn.x, ra.mn.x), fminf(la.mn.y, ra.mn.y), fminf(la.mn.z, ra.mn.z));
-        r.mx = make_float3(fmaxf(la.mx.x, ra.mx.x), fmaxf(la.mx.y, ra.mx.y), fmaxf(la.mx.z, ra.mx.z));
-        // opening geometry (solvers/src/lbvh.rs:602-611, 511-514): accept iff
-        // theta*(d - delta) >= s  <=>  d >= s/theta + delta
-        const float cx = (r.mn.x + r.mx.x) * 0.5f, cy = (r.mn.y + r.mx.y) * 0.5f,
-                    cz = (r.mn.z + r.mx.z) * 0.5f;
-        const float hx = (r.mx.x - r.mn.x) * 0.5f, hy = (r.mx.y - r.mn.y) * 0.5f,
-                    hz = (r.mx.z - r.mn.z) * 0.5f;
-        const float size = 2.0f * fmaxf(hx, fmaxf(hy, hz));
-        const float ddx = r.com.x - cx, ddy = r.com.y - cy, ddz = r.com.z - cz;
-        const float delta = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-        const float rcrit = size * inv_theta + delta;
-        const uint32_t h = s.hi[j];
-        Node nd;
-        nd.a = make_float4(r.com.x, r.com.y, r.com.z, r.mass);
-        nd.b = make_float4(r.mn.x, r.mn.y, r.mn.z, rcrit * rcrit);
-        nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(lc));
-        nd.d = make_uint4(h == n - 1 ? kEnd : s.rs[h], 0u, 0u, 0u);
-        s.node[j] = nd;
+        const uint32_t q = j - t0;
+        const uint32_t old =
+            __hip_atomic_fetch_add(&s_flag[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) break;
+        AggV ch[2];
+        const uint32_t cs[2] = {s.left[j], s.right[j]};
+        for (int c = 0; c < 2; ++c) {
+          if (cs[c] < n) {
+            ch[c] = leaf_agg(s.leaf[cs[c]]);
+          } else {
+            const uint32_t cq = cs[c] - n - t0;
+            AggV& r = ch[c];
+            r.com = make_float3(s_com[0][cq], s_com[1][cq], s_com[2][cq]);
+            r.mass = s_mass[cq];
+            r.mn = make_float3(s_mn[0][cq], s_mn[1][cq], s_mn[2][cq]);
+            r.mx = make_float3(s_mx[0][cq], s_mx[1][cq], s_mx[2][cq]);
+            for (int w = 0; w < 6; ++w) r.q[w] = QUAD ? s_q[QUAD ? w : 0][cq] : 0.f;
+          }
+        }
+        const AggV r = fold_agg<QUAD>(ch[0], ch[1]);
+        s_com[0][q] = r.com.x; s_com[1][q] = r.com.y; s_com[2][q] = r.com.z;
+        s_mass[q] = r.mass;
+        s_mn[0][q] = r.mn.x; s_mn[1][q] = r.mn.y; s_mn[2][q] = r.mn.z;
+        s_mx[0][q] = r.mx.x; s_mx[1][q] = r.mx.y; s_mx[2][q] = r.mx.z;
+        if (QUAD)
+          for (int w = 0; w < 6; ++w) s_q[QUAD ? w : 0][q] = r.q[w];
+        store_node<QUAD>(s, j, r, inv_theta);
+        u = p;
         p = s.parent_int[j];
       }
     }
+    __syncthreads();
+  }
+}
+
+template <bool QUAD>
+__global__ void __launch_bounds__(kBlock)
+k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
+  for (uint32_t sidx = blockIdx.x; sidx < nseg; sidx += gridDim.x) {
+    const SegDesc& s = segs[sidx];
+    const uint32_t n = s.n;
+    if (n >= 2) {
+      const float inv_theta = 1.0f / s.theta;
+      const unsigned long long cnt64 = s.counters[kCntRoots];
+      const uint32_t cnt = cnt64 < n ? (uint32_t)cnt64 : n;
+      for (uint32_t r = threadIdx.x; r < cnt; r += kBlock) {
+        const uint32_t u = s.roots[r];
+        uint32_t p = u < n ? s.parent_leaf[u] : s.parent_int[u - n];
+        for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {
+          const uint32_t j = p - n;
+          if (j >= n - 1) break;
+          const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (old == 0u) break;
+          __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, s.left[j]),
+                                        global_child<QUAD>(s, s.right[j]));
+          store_node<QUAD>(s, j, a, inv_theta);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          p = s.parent_int[j];
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s.counters[kCntRoots] = 0ull;
+    __syncthreads();
   }
 }
 
 // ----------------------------------------------------------------------------------
-// 7. walk (solvers/src/lbvh.rs:469-524): stackless skip-pointer traversal in DFS
-//    pre-order (left child first; after a subtree, its skip pointer), one target per
-//    lane in Morton order, Plummer-softened monopole, f32 arithmetic in
-//    origin-relative coordinates; optional Kahan-compensated accumulation.
-//    Epilogue: scatter a (x G) to the original index; resident mode also applies the
-//    closing half-kick (core/src/integrator.rs:70-73).
+// 7. walks. Accelerations are accumulated in origin-relative f32 coordinates with
+//    optional Kahan compensation; the epilogue scatters a (x G) to the original
+//    index and, in resident mode, applies the closing half-kick
+//    (core/src/integrator.rs:70-73).
 // ----------------------------------------------------------------------------------
 
 struct Acc3 {
@@ -580,90 +767,298 @@ __device__ __forceinline__ void acc_add(Acc3& a, float tx, float ty, float tz) {
   }
 }
 
+template <bool KAHAN>
+__device__ __forceinline__ void store_acc(const SegDesc& s, uint32_t k, const Acc3& a, int mode) {
+  const uint32_t n = s.n;
+  const float g = s.g;
+  const float ax = (a.x - (KAHAN ? a.cx : 0.f)) * g;
+  const float ay = (a.y - (KAHAN ? a.cy : 0.f)) * g;
+  const float az = (a.z - (KAHAN ? a.cz : 0.f)) * g;
+  const size_t i = min(s.vals[0][k], n - 1);
+  if (s.acc_out) {
+    s.acc_out[3 * i] = (double)ax;
+    s.acc_out[3 * i + 1] = (double)ay;
+    s.acc_out[3 * i + 2] = (double)az;
+  } else {
+    s.acc[3 * i] = ax;
+    s.acc[3 * i + 1] = ay;
+    s.acc[3 * i + 2] = az;
+    if (mode == 1) {
+      const float h = s.half_dt;
+      s.v[3 * i] += ax * h;
+      s.v[3 * i + 1] += ay * h;
+      s.v[3 * i + 2] += az * h;
+    }
+  }
+}
+
+// Per-target walk (solvers/src/lbvh.rs:469-524): stackless skip-pointer traversal in
+// DFS pre-order (left child first; after a subtree, its skip pointer), reference
+// acceptance test, Plummer-softened monopole, self excluded by leaf index.
+template <bool KAHAN, bool COUNT>
+__device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 me, Acc3& a,
+                                          unsigned long long& n_int,
+                                          unsigned long long& n_vis) {
+  const uint32_t n = s.n;
+  const float eps2 = s.eps2;
+  const float4* __restrict__ leaf = s.leaf;
+  const Node* __restrict__ node = s.node;
+  const uint32_t* __restrict__ rs = s.rs;
+  uint32_t u = (n == 1) ? 0u : n;
+  // each of the 2n-1 nodes is visited at most once in pre-order; the bound only
+  // guards corrupted trees
+  for (uint32_t guard = 0; u < 2 * n - 1 && guard < 2 * n; ++guard) {
+    if (COUNT) ++n_vis;
+    if (u < n) {
+      const float4 p = leaf[u];
+      const uint32_t nxt = (u == n - 1) ? kEnd : rs[u];
+      if (u != k && p.w > 0.0f) {
+        const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
+        const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const float ir = rsqrtf(r2);
+        const float w = p.w * ir * ir * ir;
+        acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
+        if (COUNT) ++n_int;
+      }
+      u = nxt;
+    } else {
+      const Node* nd = node + (u - n);
+      const float4 A = nd->a;
+      const uint32_t nxt = nd->d.x;
+      if (!(A.w > 0.0f)) {
+        u = nxt;
+        continue;
+      }
+      const float4 B = nd->b, Cc = nd->c;
+      const bool inside = (B.x <= me.x) & (me.x <= Cc.x) & (B.y <= me.y) & (me.y <= Cc.y) &
+                          (B.z <= me.z) & (me.z <= Cc.z);
+      const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      if (!inside && d2 >= B.w) {
+        const float r2 = d2 + eps2;
+        const float ir = rsqrtf(r2);
+        const float w = A.w * ir * ir * ir;
+        acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
+        if (COUNT) ++n_int;
+        u = nxt;
+      } else {
+        u = __float_as_uint(Cc.w);
+      }
+    }
+  }
+}
+
 template <bool KAHAN, bool COUNT>
 __global__ void __launch_bounds__(kBlock)
-k_walk(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
-       int mode) {
+k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
+            int mode) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     if (tl.start == 0 && threadIdx.x < 6) bbox_neutral(s.bbox, threadIdx.x);
     if (threadIdx.x >= tl.count) continue;
-    const uint32_t n = s.n;
     const uint32_t k = tl.start + threadIdx.x;
-    const float4 me = s.leaf[k];
-    const float eps2 = s.eps2;
-    const float4* __restrict__ leaf = s.leaf;
-    const Node* __restrict__ node = s.node;
-    const uint32_t* __restrict__ rs = s.rs;
     Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     unsigned long long n_int = 0, n_vis = 0;
-    uint32_t u = (n == 1) ? 0u : n;
-    // each of the 2n-1 nodes is visited at most once in pre-order; the bound only
-    // guards corrupted trees
-    for (uint32_t guard = 0; u < 2 * n - 1 && guard < 2 * n; ++guard) {
-      if (COUNT) ++n_vis;
-      if (u < n) {
-        const float4 p = leaf[u];
-        const uint32_t nxt = (u == n - 1) ? kEnd : rs[u];
-        if (u != k && p.w > 0.0f) {
+    lane_walk<KAHAN, COUNT>(s, k, s.leaf[k], a, n_int, n_vis);
+    store_acc<KAHAN>(s, k, a, mode);
+    if (COUNT) {
+      atomicAdd(&s.counters[kCntInteractions], n_int);
+      atomicAdd(&s.counters[kCntVisits], n_vis);
+    }
+  }
+}
+
+// Group walk: one wave walks the tree for 64 Morton-consecutive targets. Nodes to
+// test are popped 64 at a time from an LDS stack (one node per lane); a node is
+// accepted for the whole group when its centre of mass is at least the reference
+// acceptance radius (s/theta + delta) from every point of the group's bounding box
+// and the group box does not overlap the node box — so each target's own reference
+// test (solvers/src/lbvh.rs:506-514) holds for every accepted node. Accepted nodes
+// and reached leaves are appended to LDS interaction lists which the 64 lanes then
+// evaluate against their own targets (LDS broadcast reads, no divergence). Opened
+// nodes push both children. If the stack would overflow, the group falls back to
+// the per-target walk.
+constexpr int kStackCap = 1536;
+constexpr int kListCap = 128;
+
+template <bool KAHAN, bool COUNT, bool QUAD>
+__global__ void __launch_bounds__(kGroup)
+k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
+             int mode) {
+  constexpr int QN = QUAD ? kListCap : 1;
+  __shared__ uint32_t stk[kStackCap];
+  __shared__ float4 pl[kListCap];
+  __shared__ float4 qa[QN], qb[QN];
+  __shared__ float2 qc[QN];
+  const uint32_t lane = threadIdx.x;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const SegDesc& s = segs[tl.seg];
+    if (tl.start == 0 && lane < 6) bbox_neutral(s.bbox, lane);
+    const uint32_t n = s.n;
+    const bool valid = lane < tl.count;
+    const uint32_t k = tl.start + (valid ? lane : 0u);
+    const float4 me = s.leaf[k];
+    const float eps2 = s.eps2;
+    const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
+    const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
+    Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    unsigned long long n_int = 0, n_vis = 0, n_quad = 0;
+    uint32_t sp = 1, np = 0, nq = 0;
+    bool overflow = false;
+    if (lane == 0) stk[0] = (n == 1) ? 0u : n;
+    __syncthreads();
+    // every node is pushed at most once, so a round count above 2n only happens on a
+    // corrupted tree; the bound keeps such input from spinning forever
+    for (uint32_t round = 0; sp > 0; ++round) {
+      if (round > 2 * n + 2) {
+        overflow = true;
+        break;
+      }
+      const uint32_t m = sp < 64u ? sp : 64u;
+      const uint32_t base = sp - m;
+      const uint32_t u = lane < m ? stk[base + lane] : kEnd;
+      sp = base;
+      if (COUNT && lane == 0) n_vis += m;
+      // Classify without divergent branches: every lane loads a valid leaf and node
+      // record (index 0 stands in for the other kind) and the action is selected, so
+      // the ballots below always see the full wave.
+      // act: 0 nothing, 1 point/monopole, 2 quadrupole node, 3 open
+      const bool live = u != kEnd;
+      const bool is_leaf = live && u < n;
+      const bool is_node = live && u >= n;
+      const uint32_t li = is_leaf ? u : 0u;
+      const uint32_t ni = is_node ? u - n : 0u;
+      const float4 LP = s.leaf[li];
+      const Node& nd = s.node[n > 1 ? ni : 0u];
+      const float4 A = nd.a, B = nd.b, Cc = nd.c;
+      const uint32_t crn = nd.d.y;
+      const float ex = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
+      const float ey = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
+      const float ez = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
+      const float dmin2 = ex * ex + ey * ey + ez * ez;
+      const bool apart = (Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) | (B.y > gy1) |
+                         (Cc.z < gz0) | (B.z > gz1);
+      const bool node_live = is_node & (A.w > 0.0f);
+      const bool accept = node_live & apart & (dmin2 >= B.w);
+      const bool open = node_live & !accept;
+      const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
+      const int act = point ? 1 : (accept ? 2 : (open ? 3 : 0));
+      const float4 P = is_leaf ? LP : A;
+      float4 Q0 = make_float4(0.f, 0.f, 0.f, 0.f);
+      float2 Q1 = make_float2(0.f, 0.f);
+      if (QUAD) {
+        const float4 q1 = s.quad[2 * ni + 1];
+        Q0 = s.quad[2 * ni];
+        Q1 = make_float2(q1.x, q1.y);
+      }
+      const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
+      const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
+                               m3 = __ballot(act == 3);
+      const uint32_t c3 = (uint32_t)__popcll(m3);
+      if (sp + 2u * c3 > (uint32_t)kStackCap) {
+        overflow = true;
+        break;
+      }
+      if (act == 1) pl[np + __popcll(m1 & lt)] = P;
+      if (QUAD && act == 2) {
+        const uint32_t q = nq + __popcll(m2 & lt);
+        qa[q] = P;
+        qb[q] = Q0;
+        qc[q] = Q1;
+      }
+      if (act == 3) {
+        const uint32_t q = sp + 2u * (uint32_t)__popcll(m3 & lt);
+        stk[q] = cr;
+        stk[q + 1] = cl;  // left child popped first
+      }
+      np += (uint32_t)__popcll(m1);
+      nq += (uint32_t)__popcll(m2);
+      sp += 2u * c3;
+      __syncthreads();
+      if (np >= 64u) {
+        for (uint32_t e = 0; e < np; ++e) {
+          const float4 p = pl[e];
           const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
           const float r2 = dx * dx + dy * dy + dz * dz + eps2;
           const float ir = rsqrtf(r2);
-          const float w = p.w * ir * ir * ir;
+          const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
           acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
-          if (COUNT) ++n_int;
         }
-        u = nxt;
-      } else {
-        const Node* nd = node + (u - n);
-        const float4 A = nd->a;
-        const uint32_t nxt = nd->d.x;
-        if (!(A.w > 0.0f)) {
-          u = nxt;
-          continue;
-        }
-        const float4 B = nd->b, Cc = nd->c;
-        const bool inside = (B.x <= me.x) & (me.x <= Cc.x) & (B.y <= me.y) & (me.y <= Cc.y) &
-                            (B.z <= me.z) & (me.z <= Cc.z);
-        const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (!inside && d2 >= B.w) {
-          const float r2 = d2 + eps2;
-          const float ir = rsqrtf(r2);
-          const float w = A.w * ir * ir * ir;
-          acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
-          if (COUNT) ++n_int;
-          u = nxt;
-        } else {
-          u = __float_as_uint(Cc.w);
-        }
+        if (COUNT) n_int += np;
+        np = 0;
       }
+      if (QUAD && nq >= 64u) {
+        for (uint32_t e = 0; e < nq; ++e) {
+          const float4 A = qa[e], B = qb[e];
+          const float2 Cq = qc[e];
+          const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
+          const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+          const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
+                      ir7 = ir5 * ir2;
+          const float qx = B.x * dx + B.w * dy + Cq.x * dz;
+          const float qy = B.w * dx + B.y * dy + Cq.y * dz;
+          const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
+          const float dqd = dx * qx + dy * qy + dz * qz;
+          const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
+          acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
+        }
+        if (COUNT) n_quad += nq;
+        nq = 0;
+      }
+      __syncthreads();
     }
-    const float g = s.g;
-    const float ax = (a.x + (KAHAN ? -a.cx : 0.f)) * g;
-    const float ay = (a.y + (KAHAN ? -a.cy : 0.f)) * g;
-    const float az = (a.z + (KAHAN ? -a.cz : 0.f)) * g;
-    const size_t i = min(s.vals[0][k], n - 1);
-    if (s.acc_out) {
-      s.acc_out[3 * i] = (double)ax;
-      s.acc_out[3 * i + 1] = (double)ay;
-      s.acc_out[3 * i + 2] = (double)az;
+    if (overflow) {
+      a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      n_int = 0;
+      n_quad = 0;
+      if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
+      if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
     } else {
-      s.acc[3 * i] = ax;
-      s.acc[3 * i + 1] = ay;
-      s.acc[3 * i + 2] = az;
-      if (mode == 1) {
-        const float h = s.half_dt;
-        s.v[3 * i] += ax * h;
-        s.v[3 * i + 1] += ay * h;
-        s.v[3 * i + 2] += az * h;
+      for (uint32_t e = 0; e < np; ++e) {
+        const float4 p = pl[e];
+        const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
+        const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const float ir = rsqrtf(r2);
+        const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
+        acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
+      }
+      if (QUAD) {
+        for (uint32_t e = 0; e < nq; ++e) {
+          const float4 A = qa[e], B = qb[e];
+          const float2 Cq = qc[e];
+          const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
+          const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+          const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
+                      ir7 = ir5 * ir2;
+          const float qx = B.x * dx + B.w * dy + Cq.x * dz;
+          const float qy = B.w * dx + B.y * dy + Cq.y * dz;
+          const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
+          const float dqd = dx * qx + dy * qy + dz * qz;
+          const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
+          acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
+        }
+      }
+      if (COUNT) {
+        n_int += np;
+        n_quad += nq;
       }
     }
+    if (valid) store_acc<KAHAN>(s, k, a, mode);
     if (COUNT) {
-      atomicAdd(&s.counters[0], n_int);
-      atomicAdd(&s.counters[1], n_vis);
+      // n_int is per target (uniform over valid lanes unless the group fell back)
+      const unsigned long long vi = wave_sum_u64(valid ? n_int : 0ull);
+      const unsigned long long vq = wave_sum_u64(valid ? n_quad : 0ull);
+      const unsigned long long vv = wave_sum_u64(n_vis);
+      if (lane == 0) {
+        atomicAdd(&s.counters[kCntInteractions], vi);
+        atomicAdd(&s.counters[kCntQuad], vq);
+        atomicAdd(&s.counters[kCntVisits], vv);
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -673,18 +1068,16 @@ k_walk(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_
 //    Partial record: KE, P xyz, L xyz, sum m*x xyz, sum m  (11 doubles).
 // ----------------------------------------------------------------------------------
 
-constexpr int kDiagWords = 11;
-
 __global__ void __launch_bounds__(kBlock)
 k_diag_partial(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
                double* __restrict__ partial) {
-  __shared__ double sh[kDiagWords][kBlock / 64];
+  __shared__ double sh[kDiagRecord][kBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    double acc[kDiagWords];
-    for (int q = 0; q < kDiagWords; ++q) acc[q] = 0.0;
+    double acc[kDiagRecord];
+    for (int q = 0; q < kDiagRecord; ++q) acc[q] = 0.0;
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const size_t i = tl.start + e;
       const double m = s.m[i];
@@ -702,15 +1095,15 @@ k_diag_partial(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
       acc[9] += z * m;
       acc[10] += m;
     }
-    for (int q = 0; q < kDiagWords; ++q) {
+    for (int q = 0; q < kDiagRecord; ++q) {
       const double r = wave_sum(acc[q]);
       if (lane == 0) sh[q][w] = r;
     }
     __syncthreads();
-    if (threadIdx.x < kDiagWords) {
+    if (threadIdx.x < kDiagRecord) {
       double r = 0.0;
       for (int k = 0; k < kBlock / 64; ++k) r += sh[threadIdx.x][k];
-      partial[(size_t)t * kDiagWords + threadIdx.x] = r;
+      partial[(size_t)t * kDiagRecord + threadIdx.x] = r;
     }
     __syncthreads();
   }
@@ -719,12 +1112,12 @@ k_diag_partial(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
 __global__ void k_diag_final(const SegDesc* __restrict__ segs, uint32_t nseg,
                              const double* __restrict__ partial, double* __restrict__ out) {
   const uint32_t sidx = blockIdx.x;
-  if (sidx >= nseg || threadIdx.x >= kDiagWords) return;
+  if (sidx >= nseg || threadIdx.x >= kDiagRecord) return;
   const SegDesc& s = segs[sidx];
   double r = 0.0;
   for (uint32_t t = s.sort_tile0; t < s.sort_tile0 + s.sort_tiles; ++t)
-    r += partial[(size_t)t * kDiagWords + threadIdx.x];
-  out[(size_t)sidx * kDiagWords + threadIdx.x] = r;
+    r += partial[(size_t)t * kDiagRecord + threadIdx.x];
+  out[(size_t)sidx * kDiagRecord + threadIdx.x] = r;
 }
 
 // ----------------------------------------------------------------------------------
@@ -821,18 +1214,18 @@ __global__ void k_sum_f64(const double* __restrict__ in, uint32_t n, double* __r
   *out = r;
 }
 
+__global__ void k_seg_reset(const SegDesc* __restrict__ segs, uint32_t nseg) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nseg * 6u) return;
+  bbox_neutral(segs[q / 6u].bbox, (int)(q % 6u));
+}
+
 // ----------------------------------------------------------------------------------
 // host-side launch wrappers
 // ----------------------------------------------------------------------------------
 
 static inline uint32_t grid_for(uint32_t ntiles, uint32_t cap = 65535u) {
   return ntiles == 0 ? 1u : (ntiles < cap ? ntiles : cap);
-}
-
-__global__ void k_seg_reset(const SegDesc* __restrict__ segs, uint32_t nseg) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nseg * 6u) return;
-  bbox_neutral(segs[q / 6u].bbox, (int)(q % 6u));
 }
 
 hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg) {
@@ -852,14 +1245,20 @@ hipError_t launch_kick_drift_bbox(hipStream_t st, const SegDesc* segs, const Til
   return hipGetLastError();
 }
 
-hipError_t launch_encode(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net,
-                         uint32_t* lanes_out) {
-  hipLaunchKernelGGL(k_encode, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net, lanes_out);
+hipError_t launch_encode(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
+                         uint32_t net, uint32_t* lanes_out) {
+  if (md.key63)
+    hipLaunchKernelGGL(k_encode<uint64_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net,
+                       lanes_out);
+  else
+    hipLaunchKernelGGL(k_encode<uint32_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net,
+                       lanes_out);
   return hipGetLastError();
 }
 
-hipError_t launch_bounds_only(hipStream_t st, const SegDesc* segs, uint32_t nseg) {
-  hipLaunchKernelGGL(k_bounds, dim3((nseg + 63) / 64), dim3(64), 0, st, segs, nseg);
+hipError_t launch_bounds_only(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg) {
+  hipLaunchKernelGGL(k_bounds, dim3((nseg + 63) / 64), dim3(64), 0, st, segs, nseg,
+                     md.key63 ? 21 : 10);
   return hipGetLastError();
 }
 
@@ -868,19 +1267,28 @@ hipError_t launch_iota(hipStream_t st, const SegDesc* segs, const Tile* et, uint
   return hipGetLastError();
 }
 
-hipError_t launch_sort(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
-                       uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+template <class K>
+static void sort_passes(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
+                        uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
   int src = 0;
-  for (int pass = 0; pass < kRadixPasses; ++pass) {
+  for (int pass = 0; pass < KeyTraits<K>::passes; ++pass) {
     const int shift = pass * kRadixBits;
-    hipLaunchKernelGGL(k_radix_upsweep, dim3(grid_for(net)), dim3(kSortThreads), 0, st, segs, et,
-                       net, tile_hist, shift, src);
-    hipLaunchKernelGGL(k_radix_scan, dim3(grid_for(nseg)), dim3(kSortThreads), 0, st, segs, nseg,
+    hipLaunchKernelGGL(k_radix_upsweep<K>, dim3(grid_for(net)), dim3(kSortThreads), 0, st, segs,
+                       et, net, tile_hist, shift, src);
+    hipLaunchKernelGGL(k_radix_scan, dim3(grid_for(nseg)), dim3(kScanThreads), 0, st, segs, nseg,
                        tile_hist, tile_off);
-    hipLaunchKernelGGL(k_radix_downsweep, dim3(grid_for(net)), dim3(kSortThreads), 0, st, segs, et,
-                       net, tile_off, shift, src);
+    hipLaunchKernelGGL(k_radix_downsweep<K>, dim3(grid_for(net)), dim3(kSortThreads), 0, st, segs,
+                       et, net, tile_off, shift, src);
     src ^= 1;
   }
+}
+
+hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
+                       const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+  if (md.key63)
+    sort_passes<uint64_t>(st, segs, nseg, et, net, tile_hist, tile_off);
+  else
+    sort_passes<uint32_t>(st, segs, nseg, et, net, tile_hist, tile_off);
   return hipGetLastError();
 }
 
@@ -889,29 +1297,65 @@ hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, ui
   return hipGetLastError();
 }
 
-hipError_t launch_karras(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_karras, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
+                         uint32_t net) {
+  if (md.key63)
+    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  else
+    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
   return hipGetLastError();
 }
 
-hipError_t launch_aggregate(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_aggregate, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
+                            const Tile* at, uint32_t nat) {
+  if (md.quad) {
+    hipLaunchKernelGGL(k_aggregate_tile<true>, dim3(grid_for(nat)), dim3(kBlock), 0, st, segs, at,
+                       nat);
+    hipLaunchKernelGGL(k_aggregate_top<true>, dim3(grid_for(nseg)), dim3(kBlock), 0, st, segs,
+                       nseg);
+  } else {
+    hipLaunchKernelGGL(k_aggregate_tile<false>, dim3(grid_for(nat)), dim3(kBlock), 0, st, segs, at,
+                       nat);
+    hipLaunchKernelGGL(k_aggregate_top<false>, dim3(grid_for(nseg)), dim3(kBlock), 0, st, segs,
+                       nseg);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_walk(hipStream_t st, const SegDesc* segs, const Tile* wt, uint32_t nwt,
-                       int mode, bool kahan, bool count) {
+template <bool KAHAN, bool COUNT>
+static void walk_group_q(hipStream_t st, bool quad, const SegDesc* segs, const Tile* gt,
+                         uint32_t ngt, int mode) {
+  if (quad)
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true>), dim3(grid_for(ngt)), dim3(kGroup), 0, st,
+                       segs, gt, ngt, mode);
+  else
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false>), dim3(grid_for(ngt)), dim3(kGroup), 0,
+                       st, segs, gt, ngt, mode);
+}
+
+hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* wt,
+                       uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count) {
+  if (md.group) {
+    if (md.kahan) {
+      if (count) walk_group_q<true, true>(st, md.quad, segs, gt, ngt, mode);
+      else walk_group_q<true, false>(st, md.quad, segs, gt, ngt, mode);
+    } else {
+      if (count) walk_group_q<false, true>(st, md.quad, segs, gt, ngt, mode);
+      else walk_group_q<false, false>(st, md.quad, segs, gt, ngt, mode);
+    }
+    return hipGetLastError();
+  }
   const dim3 g(grid_for(nwt)), b(kBlock);
-  if (kahan) {
+  if (md.kahan) {
     if (count)
-      hipLaunchKernelGGL((k_walk<true, true>), g, b, 0, st, segs, wt, nwt, mode);
+      hipLaunchKernelGGL((k_walk_lane<true, true>), g, b, 0, st, segs, wt, nwt, mode);
     else
-      hipLaunchKernelGGL((k_walk<true, false>), g, b, 0, st, segs, wt, nwt, mode);
+      hipLaunchKernelGGL((k_walk_lane<true, false>), g, b, 0, st, segs, wt, nwt, mode);
   } else {
     if (count)
-      hipLaunchKernelGGL((k_walk<false, true>), g, b, 0, st, segs, wt, nwt, mode);
+      hipLaunchKernelGGL((k_walk_lane<false, true>), g, b, 0, st, segs, wt, nwt, mode);
     else
-      hipLaunchKernelGGL((k_walk<false, false>), g, b, 0, st, segs, wt, nwt, mode);
+      hipLaunchKernelGGL((k_walk_lane<false, false>), g, b, 0, st, segs, wt, nwt, mode);
   }
   return hipGetLastError();
 }

@@ -55,6 +55,10 @@ class Batch {
 
   // Rebuild tables for the given descriptors (copied to the device).
   void set_segments(const std::vector<SegDesc>& descs);
+  // Patch per-segment step parameters (dt) on the device without rebuilding tables.
+  void set_dt(double dt);
+  void set_mode(const Mode& m) { mode_ = m; }
+  const Mode& mode() const { return mode_; }
   uint32_t nseg() const { return nseg_; }
   const std::vector<SegDesc>& host_descs() const { return host_; }
   SegDesc* dev_descs() const { return segs_; }
@@ -70,26 +74,29 @@ class Batch {
   void gather();
   void karras();
   void aggregate();
-  void walk(int mode, bool kahan, bool count);
+  void walk(int mode, bool count);
   // bbox-independent tail: encode .. walk (bbox must already be accumulated)
-  void build_and_walk(int mode, bool kahan, bool count);
+  void build_and_walk(int mode, bool count);
   void diag(double* host_out /* nseg * kDiagRecord */);
 
   hipStream_t stream() const { return stream_; }
 
  private:
   hipStream_t stream_;
+  Mode mode_{};
   uint32_t nseg_ = 0;
   std::vector<SegDesc> host_;
   SegDesc* segs_ = nullptr;
-  Tile* etiles_ = nullptr;
-  Tile* wtiles_ = nullptr;
-  uint32_t n_et_ = 0, n_wt_ = 0;
+  Tile* etiles_ = nullptr;  // kSortTile elements per tile
+  Tile* atiles_ = nullptr;  // kAggTile leaves per tile (bottom-up pass)
+  Tile* wtiles_ = nullptr;  // kBlock targets per tile (per-target walk)
+  Tile* gtiles_ = nullptr;  // kGroup targets per tile (group walk)
+  uint32_t n_et_ = 0, n_at_ = 0, n_wt_ = 0, n_gt_ = 0;
   uint32_t* tile_hist_ = nullptr;
   uint32_t* tile_off_ = nullptr;
   double* diag_partial_ = nullptr;
   double* diag_out_ = nullptr;
-  size_t cap_segs_ = 0, cap_et_ = 0, cap_wt_ = 0;
+  size_t cap_segs_ = 0, cap_et_ = 0, cap_at_ = 0, cap_wt_ = 0, cap_gt_ = 0;
 };
 
 }  // namespace glx

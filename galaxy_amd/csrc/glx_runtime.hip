@@ -54,8 +54,8 @@ struct Carve {
 };
 
 struct Layout {
-  size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, left, right, pleaf, pint, rs, hi,
-      flag, bbox, bounds, counters, total;
+  size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, quad, left, right, pleaf, pint, rs, lo,
+      hi, flag, roots, bbox, bounds, counters, total;
 };
 
 Layout layout_for(uint32_t n, bool with_acc_out) {
@@ -67,22 +67,25 @@ Layout layout_for(uint32_t n, bool with_acc_out) {
   L.m = c.take<double>(N);
   L.acc = c.take<float>(3 * N);
   L.acc_out = with_acc_out ? c.take<double>(3 * N) : 0;
-  L.k0 = c.take<uint32_t>(N);
-  L.k1 = c.take<uint32_t>(N);
+  L.k0 = c.take<uint64_t>(N);
+  L.k1 = c.take<uint64_t>(N);
   L.v0 = c.take<uint32_t>(N);
   L.v1 = c.take<uint32_t>(N);
   L.leaf = c.take<float4>(N);
   L.node = c.take<Node>(N);
+  L.quad = c.take<float4>(2 * N);
   L.left = c.take<uint32_t>(N);
   L.right = c.take<uint32_t>(N);
   L.pleaf = c.take<uint32_t>(N);
   L.pint = c.take<uint32_t>(N);
   L.rs = c.take<uint32_t>(N);
+  L.lo = c.take<uint32_t>(N);
   L.hi = c.take<uint32_t>(N);
   L.flag = c.take<uint32_t>(N);
+  L.roots = c.take<uint32_t>(N);
   L.bbox = c.take<unsigned long long>(6);
   L.bounds = c.take<double>(8);
-  L.counters = c.take<unsigned long long>(4);
+  L.counters = c.take<unsigned long long>(kCounters);
   L.total = c.off;
   return L;
 }
@@ -104,9 +107,10 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   const Layout L = layout_for(cap, acc_out);
   void* blk = nullptr;
   check(hipMalloc(&blk, L.total), "hipMalloc(segment)");
-  // flags must start at zero (the bottom-up pass keeps them zero afterwards)
+  // arrival flags and counters must start at zero (kernels keep them zero afterwards)
   hipError_t e = hipMemset(static_cast<char*>(blk) + L.flag, 0, sizeof(uint32_t) * (cap ? cap : 1));
-  if (e == hipSuccess) e = hipMemset(static_cast<char*>(blk) + L.counters, 0, 32);
+  if (e == hipSuccess)
+    e = hipMemset(static_cast<char*>(blk) + L.counters, 0, kCounters * sizeof(unsigned long long));
   if (e != hipSuccess) {
     (void)hipFree(blk);
     check(e, "hipMemset(segment)");
@@ -122,24 +126,26 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   d.v = reinterpret_cast<float*>(b + L.v);
   d.m = reinterpret_cast<double*>(b + L.m);
   d.acc = reinterpret_cast<float*>(b + L.acc);
-  d.acc_out = nullptr;
-  d.keys[0] = reinterpret_cast<uint32_t*>(b + L.k0);
-  d.keys[1] = reinterpret_cast<uint32_t*>(b + L.k1);
+  d.acc_out = acc_out ? reinterpret_cast<double*>(b + L.acc_out) : nullptr;
+  d.keys[0] = b + L.k0;
+  d.keys[1] = b + L.k1;
   d.vals[0] = reinterpret_cast<uint32_t*>(b + L.v0);
   d.vals[1] = reinterpret_cast<uint32_t*>(b + L.v1);
   d.leaf = reinterpret_cast<float4*>(b + L.leaf);
   d.node = reinterpret_cast<Node*>(b + L.node);
+  d.quad = reinterpret_cast<float4*>(b + L.quad);
   d.left = reinterpret_cast<uint32_t*>(b + L.left);
   d.right = reinterpret_cast<uint32_t*>(b + L.right);
   d.parent_leaf = reinterpret_cast<uint32_t*>(b + L.pleaf);
   d.parent_int = reinterpret_cast<uint32_t*>(b + L.pint);
   d.rs = reinterpret_cast<uint32_t*>(b + L.rs);
+  d.lo = reinterpret_cast<uint32_t*>(b + L.lo);
   d.hi = reinterpret_cast<uint32_t*>(b + L.hi);
   d.flag = reinterpret_cast<uint32_t*>(b + L.flag);
+  d.roots = reinterpret_cast<uint32_t*>(b + L.roots);
   d.bbox = reinterpret_cast<unsigned long long*>(b + L.bbox);
   d.bounds = reinterpret_cast<double*>(b + L.bounds);
   d.counters = reinterpret_cast<unsigned long long*>(b + L.counters);
-  d.acc_out = acc_out ? reinterpret_cast<double*>(b + L.acc_out) : nullptr;
   desc = d;
 }
 
@@ -150,7 +156,9 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
 Batch::~Batch() {
   (void)hipFree(segs_);
   (void)hipFree(etiles_);
+  (void)hipFree(atiles_);
   (void)hipFree(wtiles_);
+  (void)hipFree(gtiles_);
   (void)hipFree(tile_hist_);
   (void)hipFree(tile_off_);
   (void)hipFree(diag_partial_);
@@ -169,24 +177,31 @@ static void grow(T*& p, size_t& cap, size_t need) {
   cap = c;
 }
 
+static void push_tiles(std::vector<Tile>& v, uint32_t seg, uint32_t n, uint32_t size) {
+  for (uint32_t st = 0; st < n; st += size)
+    v.push_back(Tile{seg, st, (n - st) < size ? n - st : size, 0});
+}
+
 void Batch::set_segments(const std::vector<SegDesc>& descs) {
   host_ = descs;
   nseg_ = (uint32_t)descs.size();
-  std::vector<Tile> et, wt;
+  std::vector<Tile> et, at, wt, gt;
   for (uint32_t s = 0; s < nseg_; ++s) {
     SegDesc& d = host_[s];
     d.sort_tile0 = (uint32_t)et.size();
-    for (uint32_t st = 0; st < d.n; st += kSortTile)
-      et.push_back(Tile{s, st, (d.n - st) < (uint32_t)kSortTile ? d.n - st : (uint32_t)kSortTile, 0});
+    push_tiles(et, s, d.n, kSortTile);
     d.sort_tiles = (uint32_t)et.size() - d.sort_tile0;
-    for (uint32_t st = 0; st < d.n; st += kBlock)
-      wt.push_back(Tile{s, st, (d.n - st) < (uint32_t)kBlock ? d.n - st : (uint32_t)kBlock, 0});
+    push_tiles(at, s, d.n, kAggTile);
+    push_tiles(wt, s, d.n, kBlock);
+    push_tiles(gt, s, d.n, kGroup);
   }
   n_et_ = (uint32_t)et.size();
+  n_at_ = (uint32_t)at.size();
   n_wt_ = (uint32_t)wt.size();
+  n_gt_ = (uint32_t)gt.size();
   size_t cs = cap_segs_;
   grow(segs_, cs, nseg_ ? nseg_ : 1);
-  if (cs != cap_segs_) {
+  if (cs != cap_segs_ || !diag_out_) {
     (void)hipFree(diag_out_);
     diag_out_ = nullptr;
     check(hipMalloc(&diag_out_, cs * kDiagRecord * sizeof(double)), "hipMalloc(diag)");
@@ -205,17 +220,30 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
     check(hipMalloc(&diag_partial_, ce * kDiagRecord * sizeof(double)), "hipMalloc(diag)");
   }
   cap_et_ = ce;
+  grow(atiles_, cap_at_, n_at_ ? n_at_ : 1);
   grow(wtiles_, cap_wt_, n_wt_ ? n_wt_ : 1);
+  grow(gtiles_, cap_gt_, n_gt_ ? n_gt_ : 1);
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "upload table");
+  };
+  up(segs_, host_.data(), nseg_ * sizeof(SegDesc));
+  up(etiles_, et.data(), n_et_ * sizeof(Tile));
+  up(atiles_, at.data(), n_at_ * sizeof(Tile));
+  up(wtiles_, wt.data(), n_wt_ * sizeof(Tile));
+  up(gtiles_, gt.data(), n_gt_ * sizeof(Tile));
+  check(hipStreamSynchronize(stream_), "set_segments");
+}
+
+void Batch::set_dt(double dt) {
+  for (auto& d : host_) {
+    d.dt = dt;
+    d.half_dt = (float)(0.5 * dt);
+  }
   if (nseg_)
     check(hipMemcpyAsync(segs_, host_.data(), nseg_ * sizeof(SegDesc), hipMemcpyHostToDevice,
-                         stream_), "upload segments");
-  if (n_et_)
-    check(hipMemcpyAsync(etiles_, et.data(), n_et_ * sizeof(Tile), hipMemcpyHostToDevice, stream_),
-          "upload tiles");
-  if (n_wt_)
-    check(hipMemcpyAsync(wtiles_, wt.data(), n_wt_ * sizeof(Tile), hipMemcpyHostToDevice, stream_),
-          "upload tiles");
-  check(hipStreamSynchronize(stream_), "set_segments");
+                         stream_),
+          "upload segments");
+  check(hipStreamSynchronize(stream_), "set_dt");
 }
 
 void Batch::reset_bbox() { check(launch_seg_reset(stream_, segs_, nseg_), "seg_reset"); }
@@ -224,27 +252,29 @@ void Batch::kick_drift_bbox() {
   check(launch_kick_drift_bbox(stream_, segs_, etiles_, n_et_), "kick_drift");
 }
 void Batch::encode(uint32_t* lanes_out) {
-  check(launch_encode(stream_, segs_, etiles_, n_et_, lanes_out), "encode");
+  check(launch_encode(stream_, mode_, segs_, etiles_, n_et_, lanes_out), "encode");
 }
-void Batch::bounds_only() { check(launch_bounds_only(stream_, segs_, nseg_), "bounds"); }
+void Batch::bounds_only() { check(launch_bounds_only(stream_, mode_, segs_, nseg_), "bounds"); }
 void Batch::iota() { check(launch_iota(stream_, segs_, etiles_, n_et_), "iota"); }
 void Batch::sort() {
-  check(launch_sort(stream_, segs_, nseg_, etiles_, n_et_, tile_hist_, tile_off_), "sort");
+  check(launch_sort(stream_, mode_, segs_, nseg_, etiles_, n_et_, tile_hist_, tile_off_), "sort");
 }
 void Batch::gather() { check(launch_gather(stream_, segs_, etiles_, n_et_), "gather"); }
-void Batch::karras() { check(launch_karras(stream_, segs_, etiles_, n_et_), "karras"); }
-void Batch::aggregate() { check(launch_aggregate(stream_, segs_, etiles_, n_et_), "aggregate"); }
-void Batch::walk(int mode, bool kahan, bool count) {
-  check(launch_walk(stream_, segs_, wtiles_, n_wt_, mode, kahan, count), "walk");
+void Batch::karras() { check(launch_karras(stream_, mode_, segs_, etiles_, n_et_), "karras"); }
+void Batch::aggregate() {
+  check(launch_aggregate(stream_, mode_, segs_, nseg_, atiles_, n_at_), "aggregate");
+}
+void Batch::walk(int mode, bool count) {
+  check(launch_walk(stream_, mode_, segs_, wtiles_, n_wt_, gtiles_, n_gt_, mode, count), "walk");
 }
 
-void Batch::build_and_walk(int mode, bool kahan, bool count) {
+void Batch::build_and_walk(int mode, bool count) {
   encode();
   sort();
   gather();
   karras();
   aggregate();
-  walk(mode, kahan, count);
+  walk(mode, count);
 }
 
 void Batch::diag(double* host_out) {

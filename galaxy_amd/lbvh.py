@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libgalaxy_lbvh.so")
 GLX_OK, GLX_EINVAL, GLX_ENOMEM, GLX_EDEVICE, GLX_EBUDGET, GLX_ENOSLOT, GLX_ENOTFOUND = range(7)
 PREC_F32, PREC_KAHAN = 0, 1
 FLAG_COUNT = 1
+WALK_REFERENCE, WALK_GROUP = 0, 1
 NO_PARENT = 0xFFFFFFFF
 
 
@@ -40,6 +41,8 @@ class Params(C.Structure):
         ("multipole", C.c_uint32),
         ("precision", C.c_uint32),
         ("flags", C.c_uint32),
+        ("morton_bits", C.c_uint32),
+        ("walk", C.c_uint32),
     ]
 
 
@@ -52,6 +55,21 @@ class Diag(C.Structure):
         ("center_of_mass", C.c_double * 3),
         ("mass", C.c_double),
     ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("point_terms", C.c_uint64),
+        ("quad_terms", C.c_uint64),
+        ("node_tests", C.c_uint64),
+        ("fallbacks", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: int(getattr(self, k)) for k, _ in self._fields_}
+        d["interactions"] = d["point_terms"] + d["quad_terms"]
+        d["node_visits"] = d["node_tests"]
+        return d
 
 
 _lib = None
@@ -76,17 +94,22 @@ def lib():
         "glx_solver_accelerations": (I, [P, S, P, P, P]),
         "glx_solver_accelerations_active": (I, [P, S, P, P, P, S, P]),
         "glx_solver_potential_energy": (I, [P, S, P, P, C.POINTER(D)]),
-        "glx_solver_stats": (I, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "glx_solver_stats": (I, [P, C.POINTER(Stats)]),
         "glx_direct_sum_f64": (I, [S, P, P, D, D, P, S, P]),
         "glx_stage_morton": (I, [S, P, P, P, P]),
+        "glx_stage_morton63": (I, [S, P, P, P, P]),
         "glx_stage_sort": (I, [S, P, P, P]),
+        "glx_stage_sort63": (I, [S, P, P, P]),
         "glx_stage_karras": (I, [S, P, P, P, P]),
-        "glx_stage_aggregate": (I, [S, P, P, P, P, P, P, P, P]),
+        "glx_stage_karras63": (I, [S, P, P, P, P]),
+        "glx_stage_aggregate": (I, [S, P, P, P, P, P, P, P, P, P]),
         "glx_sys_create": (I, [C.POINTER(Params), C.POINTER(P)]),
         "glx_sys_destroy": (I, [P]),
         "glx_sys_upload": (I, [P, S, P, P, P]),
         "glx_sys_step": (I, [P, D, C.c_uint64]),
         "glx_sys_snapshot": (I, [P, P, P]),
+        "glx_sys_step_timed": (I, [P, D, C.c_uint64, P]),
+        "glx_sys_count": (I, [P, C.POINTER(Stats)]),
         "glx_sys_snapshot_accel": (I, [P, P]),
         "glx_sys_time": (I, [P, C.POINTER(D)]),
         "glx_sys_len": (I, [P, C.POINTER(S)]),
@@ -126,9 +149,10 @@ def device_info():
     return {"name": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
 
 
-def _params(g, softening, theta, precision=PREC_F32, count=False) -> Params:
-    return Params(float(g), float(softening), float(theta), 1, 0, int(precision),
-                  FLAG_COUNT if count else 0)
+def _params(g, softening, theta, precision=PREC_F32, count=False, morton_bits=30,
+            walk=WALK_REFERENCE, multipole=0) -> Params:
+    return Params(float(g), float(softening), float(theta), 1, int(multipole), int(precision),
+                  FLAG_COUNT if count else 0, int(morton_bits), int(walk))
 
 
 @dataclass
@@ -149,11 +173,12 @@ class Lbvh:
     solvers/src/lbvh.rs:653). accelerations() mirrors ForceSolver::accelerations."""
 
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
-                 count: bool = False):
+                 count: bool = False, morton_bits: int = 30, walk: int = WALK_REFERENCE,
+                 multipole: int = 0):
         self.g, self.softening, self.theta = float(g), float(softening), float(theta)
         h = C.c_void_p()
-        _check(lib().glx_solver_create(C.byref(_params(g, softening, theta, precision, count)),
-                                       C.byref(h)))
+        p = _params(g, softening, theta, precision, count, morton_bits, walk, multipole)
+        _check(lib().glx_solver_create(C.byref(p), C.byref(h)))
         self._h = h
 
     def close(self):
@@ -188,10 +213,10 @@ class Lbvh:
                                                  C.byref(out)))
         return out.value
 
-    def stats(self):
-        a, b = C.c_uint64(0), C.c_uint64(0)
-        _check(lib().glx_solver_stats(self._h, C.byref(a), C.byref(b)))
-        return {"interactions": a.value, "node_visits": b.value}
+    def stats(self) -> dict:
+        st = Stats()
+        _check(lib().glx_solver_stats(self._h, C.byref(st)))
+        return st.as_dict()
 
 
 def direct_sum_f64(pos, mass, g: float, softening: float, targets) -> np.ndarray:
@@ -217,19 +242,31 @@ class MortonOut:
     codes: np.ndarray
 
 
-def morton(pos) -> MortonOut:
-    """GPU counterpart of reference_morton (solvers/src/lbvh.rs:105-129)."""
+def morton(pos, bits: int = 30) -> MortonOut:
+    """GPU counterpart of reference_morton (solvers/src/lbvh.rs:105-129); bits=63 for
+    the 2^21-cells-per-axis refinement."""
     pos = _f64(pos, (-1, 3))
     n = pos.shape[0]
     b = np.zeros(5)
     lanes = np.zeros((n, 3), np.uint32)
-    codes = np.zeros(n, np.uint32)
-    _check(lib().glx_stage_morton(n, _p(pos), _p(b), _p(lanes), _p(codes)))
+    if bits == 63:
+        codes = np.zeros(n, np.uint64)
+        _check(lib().glx_stage_morton63(n, _p(pos), _p(b), _p(lanes), _p(codes)))
+    else:
+        codes = np.zeros(n, np.uint32)
+        _check(lib().glx_stage_morton(n, _p(pos), _p(b), _p(lanes), _p(codes)))
     return MortonOut(b[:3].copy(), float(b[3]), float(b[4]), lanes, codes)
 
 
 def sort(codes):
-    """GPU counterpart of reference_sort (solvers/src/lbvh.rs:143-147)."""
+    """GPU counterpart of reference_sort (solvers/src/lbvh.rs:143-147); uint64 codes
+    select the 63-bit key sort."""
+    if np.asarray(codes).dtype == np.uint64:
+        c = np.ascontiguousarray(codes, dtype=np.uint64)
+        order = np.zeros(c.size, np.uint32)
+        sc = np.zeros(c.size, np.uint64)
+        _check(lib().glx_stage_sort63(c.size, _p(c), _p(order), _p(sc)))
+        return order, sc
     codes = _u32(codes)
     order = np.zeros(codes.size, np.uint32)
     sc = np.zeros(codes.size, np.uint32)
@@ -238,19 +275,23 @@ def sort(codes):
 
 
 def karras(sorted_codes):
-    """GPU counterpart of reference_karras (solvers/src/lbvh.rs:276-306)."""
-    sc = _u32(sorted_codes)
+    """GPU counterpart of reference_karras (solvers/src/lbvh.rs:276-306); uint64 codes
+    select the 63-bit build."""
+    wide = np.asarray(sorted_codes).dtype == np.uint64
+    sc = np.ascontiguousarray(sorted_codes, dtype=np.uint64) if wide else _u32(sorted_codes)
     n = sc.size
     left = np.zeros(max(n - 1, 0), np.uint32)
     right = np.zeros(max(n - 1, 0), np.uint32)
     parent = np.zeros(max(2 * n - 1, 0), np.uint32)
-    _check(lib().glx_stage_karras(n, _p(sc), _p(left), _p(right), _p(parent)))
+    f = lib().glx_stage_karras63 if wide else lib().glx_stage_karras
+    _check(f(n, _p(sc), _p(left), _p(right), _p(parent)))
     return left, right, parent
 
 
-def aggregate(sorted_codes, sorted_pos, sorted_mass):
+def aggregate(sorted_codes, sorted_pos, sorted_mass, quad: bool = False):
     """GPU counterpart of reference_aggregate (solvers/src/lbvh.rs:329-379): per unified
-    node AABB/com/mass in f32 relative to `origin` (f64)."""
+    node AABB/com/mass in f32 relative to `origin` (f64); with quad, also the traceless
+    quadrupole {xx, yy, zz, xy, xz, yz} about each node's centre of mass."""
     sc = _u32(sorted_codes)
     sp = _f64(sorted_pos, (-1, 3))
     sm = _f64(sorted_mass)
@@ -263,10 +304,13 @@ def aggregate(sorted_codes, sorted_pos, sorted_mass):
         "com": np.zeros((t, 3), np.float32),
         "mass": np.zeros(t, np.float32),
     }
+    q = np.zeros((t, 6), np.float32) if quad else None
     _check(lib().glx_stage_aggregate(n, _p(sc), _p(sp), _p(sm), _p(origin),
                                      _p(out["aabb_min"]), _p(out["aabb_max"]), _p(out["com"]),
-                                     _p(out["mass"])))
+                                     _p(out["mass"]), None if q is None else _p(q)))
     out["origin"] = origin
+    if quad:
+        out["quad"] = q
     return out
 
 
@@ -277,9 +321,11 @@ class GpuResidentLeapfrog:
     """Resident KDK stepping (gpu/src/gpu_resident.rs:1246-1968): upload -> step* ->
     snapshot. Positions FP64 on the device, velocities and forces FP32."""
 
-    def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32):
+    def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
+                 morton_bits: int = 30, walk: int = WALK_REFERENCE, multipole: int = 0):
         h = C.c_void_p()
-        _check(lib().glx_sys_create(C.byref(_params(g, softening, theta, precision)), C.byref(h)))
+        p = _params(g, softening, theta, precision, False, morton_bits, walk, multipole)
+        _check(lib().glx_sys_create(C.byref(p), C.byref(h)))
         self._h = h
         self._mass = np.zeros(0)
 
@@ -309,6 +355,20 @@ class GpuResidentLeapfrog:
 
     def step_many(self, dt: float, steps: int):
         _check(lib().glx_sys_step(self._h, float(dt), int(steps)))
+
+    PHASES = ("kick_drift_bbox", "encode", "sort", "gather", "karras", "aggregate", "walk")
+
+    def step_timed(self, dt: float, steps: int) -> dict:
+        """Direct launches with HIP events between phases; returns device ms per phase
+        summed over the steps."""
+        ph = np.zeros(len(self.PHASES))
+        _check(lib().glx_sys_step_timed(self._h, float(dt), int(steps), _p(ph)))
+        return dict(zip(self.PHASES, ph.tolist()))
+
+    def count(self) -> dict:
+        st = Stats()
+        _check(lib().glx_sys_count(self._h, C.byref(st)))
+        return st.as_dict()
 
     def snapshot(self) -> State:
         n = len(self)

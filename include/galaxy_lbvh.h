@@ -38,16 +38,22 @@ extern "C" {
 /* flags */
 #define GLX_FLAG_COUNT 1u /* count walk interactions / node visits (glx_solver_stats) */
 
+/* walk */
+#define GLX_WALK_REFERENCE 0 /* per-target walk, the reference acceptance test */
+#define GLX_WALK_GROUP 1     /* wave-cooperative group walk (conservative group test) */
+
 /* Solver parameters: the (g, softening, theta) of Lbvh::new
  * (solvers/src/lbvh.rs:652-659) plus this implementation's knobs. */
 typedef struct glx_params {
-  double g;           /* gravitational constant */
-  double softening;   /* Plummer softening length eps (> = 0) */
-  double theta;       /* opening angle (> 0) */
-  uint32_t leaf_size; /* bodies per leaf; must be 1 (the reference's LBVH leaf) */
-  uint32_t multipole; /* 0 = monopole (the reference's expansion) */
-  uint32_t precision; /* GLX_PREC_* */
-  uint32_t flags;     /* GLX_FLAG_* */
+  double g;             /* gravitational constant */
+  double softening;     /* Plummer softening length eps (> = 0) */
+  double theta;         /* opening angle (> 0) */
+  uint32_t leaf_size;   /* bodies per leaf; must be 1 (the reference's LBVH leaf) */
+  uint32_t multipole;   /* 0 = monopole (the reference's expansion), 2 = + quadrupole */
+  uint32_t precision;   /* GLX_PREC_* */
+  uint32_t flags;       /* GLX_FLAG_* */
+  uint32_t morton_bits; /* 30 (reference, also 0) or 63 (2^21 cells per axis) */
+  uint32_t walk;        /* GLX_WALK_*; quadrupoles need GLX_WALK_GROUP */
 } glx_params;
 
 /* FP64 conservation diagnostics (core/src/diagnostics.rs:8-68). */
@@ -59,6 +65,16 @@ typedef struct glx_diag {
   double center_of_mass[3];  /* sum m r / sum m */
   double mass;               /* sum m */
 } glx_diag;
+
+/* Walk statistics (measurement hook; no reference counterpart). Terms are summed over
+ * targets: one point-mass term is one leaf body or monopole node acting on one target
+ * (the reference's interaction, solvers/src/lbvh.rs:490-517). */
+typedef struct glx_stats {
+  uint64_t point_terms; /* leaf-body and monopole-node terms */
+  uint64_t quad_terms;  /* quadrupole-node terms */
+  uint64_t node_tests;  /* nodes tested (per-target walk: per target; group walk: per group) */
+  uint64_t fallbacks;   /* group walks that fell back to the per-target walk */
+} glx_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *glx_last_error(void);
@@ -73,8 +89,8 @@ int glx_device_info(char *name, size_t name_len, int *compute_units, uint64_t *h
 typedef struct glx_solver glx_solver;
 
 /* Lbvh::new / GpuLbvhFused::new (solvers/src/lbvh.rs:653; gpu/src/gpu_lbvh_fused.rs:50).
- * GLX_EINVAL for theta <= 0, softening < 0, non-finite g/eps/theta, leaf_size != 1,
- * multipole != 0; GLX_EDEVICE without a GPU. */
+ * GLX_EINVAL for theta <= 0, softening < 0, non-finite g/eps/theta, leaf_size > 1,
+ * unknown multipole/precision/morton_bits/walk; GLX_EDEVICE without a GPU. */
 int glx_solver_create(const glx_params *params, glx_solver **out);
 int glx_solver_destroy(glx_solver *solver);
 
@@ -96,10 +112,8 @@ int glx_solver_accelerations_active(glx_solver *solver, size_t n, const double *
 int glx_solver_potential_energy(glx_solver *solver, size_t n, const double *pos,
                                 const double *mass, double *out);
 
-/* Walk statistics of the last accelerations call (needs GLX_FLAG_COUNT): evaluated
- * terms (accepted nodes + leaf bodies other than the target) and node visits. No
- * reference counterpart (measurement hook). */
-int glx_solver_stats(glx_solver *solver, uint64_t *interactions, uint64_t *node_visits);
+/* Walk statistics of the last accelerations call (needs GLX_FLAG_COUNT). */
+int glx_solver_stats(glx_solver *solver, glx_stats *out);
 
 /* Exact FP64 direct sum (solvers/src/direct_sum.rs:25-45 pair kernel) onto a subset
  * of targets against all n sources, self excluded; acc_out holds 3 * n_targets. The
@@ -118,23 +132,32 @@ int glx_direct_sum_f64(size_t n, const double *pos, const double *mass, double g
  * lanes (3n, may be NULL) and codes (n). Bit-exact with the reference (f64). n > 0. */
 int glx_stage_morton(size_t n, const double *pos, double *bounds_out, uint32_t *lanes,
                      uint32_t *codes);
+/* 63-bit variant (2^21 cells per axis): the reference's deferred refinement
+ * (solvers/src/lbvh.rs:29-31). */
+int glx_stage_morton63(size_t n, const double *pos, double *bounds_out, uint32_t *lanes,
+                       uint64_t *codes);
 
 /* reference_sort (solvers/src/lbvh.rs:143-147): order sorting by (code, index), and
  * the sorted codes. Codes must be < 2^32 (30-bit Morton codes in practice). */
 int glx_stage_sort(size_t n, const uint32_t *codes, uint32_t *order, uint32_t *sorted_codes);
+int glx_stage_sort63(size_t n, const uint64_t *codes, uint32_t *order, uint64_t *sorted_codes);
 
 /* reference_karras (solvers/src/lbvh.rs:276-306): left/right (n-1) and parent
  * (2n-1, unified index; parent[root] = 0xFFFFFFFF). n > 0. */
 int glx_stage_karras(size_t n, const uint32_t *sorted_codes, uint32_t *left, uint32_t *right,
                      uint32_t *parent);
+int glx_stage_karras63(size_t n, const uint64_t *sorted_codes, uint32_t *left, uint32_t *right,
+                       uint32_t *parent);
 
 /* reference_aggregate (solvers/src/lbvh.rs:329-379) as computed by the device: per
  * unified node (2n-1) AABB min/max, centre of mass (3 floats each) and mass, in f32
  * coordinates relative to origin_out (the root cube centre of the sorted positions,
- * f64). n > 0. */
+ * f64). quad (may be NULL) receives 6 floats per node, the traceless quadrupole
+ * {xx, yy, zz, xy, xz, yz} = sum m (3 d d^T - |d|^2 I) about the node's centre of
+ * mass (the reference is monopole-only, solvers/src/barnes_hut.rs:33). n > 0. */
 int glx_stage_aggregate(size_t n, const uint32_t *sorted_codes, const double *sorted_pos,
                         const double *sorted_mass, double *origin_out, float *aabb_min,
-                        float *aabb_max, float *com, float *mass);
+                        float *aabb_max, float *com, float *mass, float *quad);
 
 /* ---------------------------------------------------------------------------------
  * Resident stepper: GpuResidentLeapfrog (gpu/src/gpu_resident.rs:1246-1968). State
@@ -151,6 +174,15 @@ int glx_sys_upload(glx_sys *sys, size_t n, const double *pos, const double *vel,
                    const double *mass);
 /* step / step_many (gpu/src/gpu_resident.rs:1823,1848): k KDK steps of dt. */
 int glx_sys_step(glx_sys *sys, double dt, uint64_t k);
+/* As glx_sys_step, launched without the graph and with a HIP event between phases on
+ * the system's stream; phase_ms[7] accumulates the device time of {kick-drift+bbox,
+ * encode, sort, gather, Karras, aggregate, walk+kick}. The reference times only the
+ * host side of step_many (gpu/tests/gpu_resident.rs:484-544); this is its device-side
+ * measurement hook. */
+int glx_sys_step_timed(glx_sys *sys, double dt, uint64_t k, double *phase_ms);
+/* Rebuild at the current positions and count walk interactions / node visits
+ * (measurement hook; no reference counterpart). Leaves the cached a(x) unchanged. */
+int glx_sys_count(glx_sys *sys, glx_stats *out);
 /* snapshot (gpu/src/gpu_resident.rs:1878): pos/vel (3n each, either may be NULL). */
 int glx_sys_snapshot(glx_sys *sys, double *pos, double *vel);
 /* snapshot_accel (gpu/src/gpu_resident.rs:1655): cached a (3n doubles). */

@@ -181,6 +181,135 @@ ORC_API int orc_reference_karras(size_t n_, const uint32_t *codes, uint32_t *lef
     return 0;
 }
 
+/* ---------------------------------------------------------------------------------
+ * 63-bit Morton variant: the reference's deferred resolution refinement
+ * (solvers/src/lbvh.rs:29-31 "63-bit (2^21 per axis, two-word sort on the GPU)").
+ * Same root-cube convention as reference_morton (:110-120) with cells = 2^21 per axis;
+ * the key pair is (code63, index) and delta extends into the index bits past 64.
+ * --------------------------------------------------------------------------------- */
+
+/* spread the low 21 bits of v three apart */
+ORC_API uint64_t orc_expand21(uint64_t v) {
+    uint64_t x = v & 0x1fffffull;
+    uint64_t out = 0;
+    for (int b = 0; b < 21; ++b) out |= ((x >> b) & 1ull) << (3 * b);
+    return out;
+}
+
+static inline uint32_t quant_lane21(double v) {
+    double f = rmax(floor(v), 0.0);
+    const double top = (double)((1u << 21) - 1u);
+    return f >= top ? (1u << 21) - 1u : (uint32_t)f;
+}
+
+ORC_API int orc_reference_morton63(size_t n, const double *pos, double *bounds_out,
+                                   uint32_t *lanes, uint64_t *codes) {
+    if (n == 0) return -1;
+    double lo[3] = {pos[0], pos[1], pos[2]};
+    double hi[3] = {pos[0], pos[1], pos[2]};
+    for (size_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = rmin(lo[a], pos[3 * i + a]);
+            hi[a] = rmax(hi[a], pos[3 * i + a]);
+        }
+    }
+    double center[3], ext[3];
+    for (int a = 0; a < 3; ++a) {
+        center[a] = (lo[a] + hi[a]) * 0.5;
+        ext[a] = hi[a] - lo[a];
+    }
+    double maxe = rmax(ext[0], rmax(ext[1], ext[2]));
+    double half = rmax(0.5 * maxe, 1e-12) * (1.0 + 1e-9);
+    double bmin[3];
+    for (int a = 0; a < 3; ++a) bmin[a] = center[a] - half;
+    double size = 2.0 * half;
+    double scale = (double)(1u << 21) / size;
+    if (bounds_out) {
+        bounds_out[0] = bmin[0];
+        bounds_out[1] = bmin[1];
+        bounds_out[2] = bmin[2];
+        bounds_out[3] = size;
+        bounds_out[4] = scale;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t l[3];
+        for (int a = 0; a < 3; ++a) l[a] = quant_lane21((pos[3 * i + a] - bmin[a]) * scale);
+        if (lanes) {
+            lanes[3 * i + 0] = l[0];
+            lanes[3 * i + 1] = l[1];
+            lanes[3 * i + 2] = l[2];
+        }
+        codes[i] = orc_expand21(l[0]) | (orc_expand21(l[1]) << 1) | (orc_expand21(l[2]) << 2);
+    }
+    return 0;
+}
+
+typedef struct {
+    uint64_t code;
+    uint32_t idx;
+} key63_t;
+
+static int cmp_key63(const void *a, const void *b) {
+    const key63_t *x = (const key63_t *)a, *y = (const key63_t *)b;
+    if (x->code != y->code) return x->code < y->code ? -1 : 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+ORC_API int orc_reference_sort63(size_t n, const uint64_t *codes, uint32_t *order) {
+    if (n == 0) return 0;
+    key63_t *k = (key63_t *)malloc(n * sizeof(key63_t));
+    if (!k) return -2;
+    for (size_t i = 0; i < n; ++i) {
+        k[i].code = codes[i];
+        k[i].idx = (uint32_t)i;
+    }
+    qsort(k, n, sizeof(key63_t), cmp_key63);
+    for (size_t i = 0; i < n; ++i) order[i] = k[i].idx;
+    free(k);
+    return 0;
+}
+
+static inline int64_t delta63(const uint64_t *codes, int64_t n, int64_t a, int64_t b) {
+    if (b < 0 || b >= n) return -1;
+    uint64_t ca = codes[a], cb = codes[b];
+    if (ca == cb) return 64 + clz32((uint32_t)a ^ (uint32_t)b);
+    return __builtin_clzll(ca ^ cb);
+}
+
+/* solvers/src/lbvh.rs:195-240 over 63-bit codes */
+ORC_API int orc_reference_karras63(size_t n_, const uint64_t *codes, uint32_t *left,
+                                   uint32_t *right, uint32_t *parent) {
+    if (n_ == 0) return -1;
+    int64_t n = (int64_t)n_;
+    for (int64_t u = 0; u < 2 * n - 1; ++u) parent[u] = NO_PARENT;
+    for (int64_t i = 0; i < n - 1; ++i) {
+        int64_t dir = delta63(codes, n, i, i + 1) > delta63(codes, n, i, i - 1) ? 1 : -1;
+        int64_t delta_min = delta63(codes, n, i, i - dir);
+        int64_t l_max = 2;
+        while (delta63(codes, n, i, i + l_max * dir) > delta_min) l_max *= 2;
+        int64_t l = 0;
+        for (int64_t t = l_max / 2; t > 0; t /= 2)
+            if (delta63(codes, n, i, i + (l + t) * dir) > delta_min) l += t;
+        int64_t j = i + l * dir;
+        int64_t delta_node = delta63(codes, n, i, j);
+        int64_t s = 0, t = l;
+        for (;;) {
+            t = (t + 1) / 2;
+            if (delta63(codes, n, i, i + (s + t) * dir) > delta_node) s += t;
+            if (t <= 1) break;
+        }
+        int64_t gamma = i + s * dir + (dir < 0 ? dir : 0);
+        int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+        uint32_t lu = (lo == gamma) ? (uint32_t)gamma : (uint32_t)(n + gamma);
+        uint32_t ru = (hi == gamma + 1) ? (uint32_t)(gamma + 1) : (uint32_t)(n + gamma + 1);
+        left[i] = lu;
+        right[i] = ru;
+        parent[lu] = (uint32_t)(n + i);
+        parent[ru] = (uint32_t)(n + i);
+    }
+    return 0;
+}
+
 /* One subtree aggregate (solvers/src/lbvh.rs:382-388). */
 typedef struct {
     double mass, com[3], mn[3], mx[3];

@@ -48,6 +48,68 @@ def morton(pos: np.ndarray):
     return {"bmin": bmin, "size": size, "scale": scale}, lanes, codes.astype(np.uint32)
 
 
+def morton63(pos: np.ndarray):
+    """63-bit variant: 2^21 cells per axis, same cube convention."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    lo = np.fmin.reduce(pos, axis=0)
+    hi = np.fmax.reduce(pos, axis=0)
+    center = (lo + hi) * 0.5
+    ext = hi - lo
+    maxe = max(ext[0], ext[1], ext[2])
+    half = np.float64(max(np.float64(0.5) * maxe, np.float64(1e-12))) * (np.float64(1.0) + np.float64(1e-9))
+    bmin = center - half
+    size = np.float64(2.0) * half
+    scale = np.float64(2.0 ** 21) / size
+    v = (pos - bmin) * scale
+    f = np.fmax(np.floor(v), 0.0)
+    top = float((1 << 21) - 1)
+    lanes = np.where(f >= top, top, f).astype(np.uint64)
+    codes = np.zeros(lanes.shape[0], np.uint64)
+    for b in range(21):
+        for a in range(3):
+            codes |= ((lanes[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + a)
+    return {"bmin": bmin, "size": size, "scale": scale}, lanes.astype(np.uint32), codes
+
+
+def radix_tree_bruteforce63(sorted_codes):
+    """Brute-force radix tree over 96-bit augmented keys (code63 << 32 | position)."""
+    sc = [int(c) for c in np.asarray(sorted_codes, dtype=np.uint64)]
+    n = len(sc)
+    keys = [(c << 32) | k for k, c in enumerate(sc)]
+    left = np.zeros(max(n - 1, 0), np.uint32)
+    right = np.zeros(max(n - 1, 0), np.uint32)
+    parent = np.full(2 * n - 1, NO_PARENT, np.uint32)
+    if n == 1:
+        return left, right, parent
+
+    def clz96(x):
+        return 96 - x.bit_length()
+
+    stack = [(0, n - 1, 0)]
+    while stack:
+        lo, hi, idx = stack.pop()
+        d_node = clz96(keys[lo] ^ keys[hi])
+        g = lo
+        for k in range(lo, hi):
+            if clz96(keys[lo] ^ keys[k]) > d_node:
+                g = k
+        if g == lo:
+            lu = g
+        else:
+            lu = n + g
+            stack.append((lo, g, g))
+        if g + 1 == hi:
+            ru = g + 1
+        else:
+            ru = n + g + 1
+            stack.append((g + 1, hi, g + 1))
+        left[idx] = lu
+        right[idx] = ru
+        parent[lu] = n + idx
+        parent[ru] = n + idx
+    return left, right, parent
+
+
 def stable_order(codes: np.ndarray) -> np.ndarray:
     return np.argsort(np.asarray(codes, dtype=np.uint32), kind="stable").astype(np.uint32)
 

@@ -71,6 +71,11 @@ def lib():
         L.orc_diagnostics.argtypes = [S, P, P, P, P]
         L.orc_diagnostics.restype = None
         L.orc_kdk_run.argtypes = [S, P, P, P, D, D, D, D, S, C.c_int, P, C.c_int]
+        L.orc_expand21.restype = C.c_uint64
+        L.orc_expand21.argtypes = [C.c_uint64]
+        L.orc_reference_morton63.argtypes = [S, P, P, P, P]
+        L.orc_reference_sort63.argtypes = [S, P, P]
+        L.orc_reference_karras63.argtypes = [S, P, P, P, P]
         assert L.orc_node_size() == NODE_DTYPE.itemsize
         _lib = L
     return _lib
@@ -109,6 +114,38 @@ def reference_morton(pos):
     codes = np.zeros(n, np.uint32)
     lib().orc_reference_morton(n, _p(pos), _p(bounds), _p(lanes), _p(codes))
     return {"bmin": bounds[:3].copy(), "size": bounds[3], "scale": bounds[4]}, lanes, codes
+
+
+def reference_morton63(pos):
+    """63-bit variant of solvers/src/lbvh.rs:105-129 (2^21 cells per axis)."""
+    pos = _f64(pos).reshape(-1, 3)
+    n = pos.shape[0]
+    if n == 0:
+        raise ValueError("reference_morton63 requires a non-empty system")
+    bounds = np.zeros(5)
+    lanes = np.zeros((n, 3), np.uint32)
+    codes = np.zeros(n, np.uint64)
+    lib().orc_reference_morton63(n, _p(pos), _p(bounds), _p(lanes), _p(codes))
+    return {"bmin": bounds[:3].copy(), "size": bounds[3], "scale": bounds[4]}, lanes, codes
+
+
+def reference_sort63(codes) -> np.ndarray:
+    codes = np.ascontiguousarray(codes, dtype=np.uint64)
+    order = np.zeros(codes.size, np.uint32)
+    lib().orc_reference_sort63(codes.size, _p(codes), _p(order))
+    return order
+
+
+def reference_karras63(sorted_codes):
+    sc = np.ascontiguousarray(sorted_codes, dtype=np.uint64)
+    n = sc.size
+    if n == 0:
+        raise ValueError("reference_karras63 requires a non-empty system")
+    left = np.zeros(max(n - 1, 0), np.uint32)
+    right = np.zeros(max(n - 1, 0), np.uint32)
+    parent = np.zeros(2 * n - 1, np.uint32)
+    lib().orc_reference_karras63(n, _p(sc), _p(left), _p(right), _p(parent))
+    return left, right, parent
 
 
 def reference_sort(codes) -> np.ndarray:

@@ -1,0 +1,203 @@
+"""GPU tests of the throughput modes: 63-bit Morton keys, quadrupole moments and the
+wave-cooperative group walk.
+
+- 63-bit codes, sort and topology are bit-exact with the oracle's 63-bit restatement
+  (oracle/lbvh_oracle.c orc_reference_morton63/sort63/karras63, itself pinned against an
+  independent numpy restatement and a brute-force radix tree in tests/test_oracle.py).
+- Quadrupole moments (no reference counterpart: the reference is monopole-only,
+  solvers/src/barnes_hut.rs:33) are checked against brute-force sums over each node's
+  leaves, and through the force error they buy against an FP64 direct sum.
+- The group walk accepts a node only when the reference acceptance test holds for every
+  point of the group's box, so at theta -> 0 it is the direct sum, and at finite theta
+  its error vs FP64 must not exceed the reference per-target walk's.
+Stated tolerances are in each test.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import galaxy_amd as glx
+from galaxy_amd import ic
+from oracle import oracle as O
+from tests import inputs as I
+
+pytestmark = pytest.mark.gpu
+G, EPS = 1.0, 0.05
+FAST = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
+GROUP_MONO = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=0)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+# ------------------------------------------------------------------ 63-bit build stages
+
+
+def assert_63_exact(pos, label):
+    say("stages63", label, pos.shape)
+    g = glx.morton(pos, bits=63)
+    b, lanes, codes = O.reference_morton63(pos)
+    assert np.array_equal(g.codes, codes), f"{label}: codes"
+    assert np.array_equal(g.lanes, lanes), f"{label}: lanes"
+    assert g.scale == b["scale"] and np.array_equal(g.bmin, b["bmin"])
+    order, sc = glx.sort(g.codes)
+    want = O.reference_sort63(codes)
+    assert np.array_equal(order, want), f"{label}: order"
+    assert np.array_equal(sc, codes[want])
+    l, r, p = glx.karras(sc)
+    lw, rw, pw = O.reference_karras63(codes[want])
+    assert np.array_equal(l, lw) and np.array_equal(r, rw) and np.array_equal(p, pw), label
+
+
+def test_63bit_stages_bit_exact():
+    for seed in (1, 7):
+        assert_63_exact(I.cloud(seed, 8000, 3.0), f"cloud{seed}")
+    assert_63_exact(I.cloud(3, 8000, 3.0, (1.0e6, 1.0e6, 1.0e6)), "cloud_1e6")
+    assert_63_exact(np.tile(np.array([[1.0, 1.0, 1.0]]), (300, 1)), "coincident")
+    pos, _, _ = I.plummer_numpy(16384, seed=4)
+    assert_63_exact(pos, "plummer16k")
+    pos, _, _, _ = ic.collision(1 << 17, 3)
+    assert_63_exact(pos, "collision128k")
+
+
+def test_63bit_stages_bit_exact_million():
+    rng = np.random.default_rng(8)
+    assert_63_exact(rng.standard_normal((1 << 20, 3)) * 5.0, "gauss_1M")
+
+
+# ------------------------------------------------------------------ quadrupole moments
+
+
+def test_quadrupole_moments_match_bruteforce():
+    pos = I.cloud(21, 1500, 3.0)
+    mass = I.masses(22, 1500)
+    _, _, codes = O.reference_morton(pos)
+    order = O.reference_sort(codes)
+    sc, sp, sm = codes[order], pos[order], mass[order]
+    g = glx.aggregate(sc, sp, sm, quad=True)
+    l, r, _ = O.reference_karras(sc)
+    from oracle import numpy_ref as R
+
+    sub = R.subtree_leaves(l, r, sc.size)
+    leaves = (sp - g["origin"]).astype(np.float32).astype(np.float64)
+    m32 = sm.astype(np.float32).astype(np.float64)
+    worst = 0.0
+    for u, idx in sub.items():
+        if u < sc.size:
+            assert np.all(g["quad"][u] == 0.0)
+            continue
+        x, m = leaves[idx], m32[idx]
+        com = (x * m[:, None]).sum(0) / m.sum()
+        d = x - com
+        d2 = (d * d).sum(1)
+        q = np.array([
+            (m * (3 * d[:, 0] ** 2 - d2)).sum(), (m * (3 * d[:, 1] ** 2 - d2)).sum(),
+            (m * (3 * d[:, 2] ** 2 - d2)).sum(), (m * 3 * d[:, 0] * d[:, 1]).sum(),
+            (m * 3 * d[:, 0] * d[:, 2]).sum(), (m * 3 * d[:, 1] * d[:, 2]).sum(),
+        ])
+        scale = (m * d2).sum() + 1e-30
+        err = np.abs(g["quad"][u].astype(np.float64) - q).max() / scale
+        worst = max(worst, err)
+    say("quadrupole worst rel err", worst)
+    assert worst < 1e-4
+
+
+# ------------------------------------------------------------------ group walk
+
+
+def test_group_walk_theta_zero_is_direct_sum():
+    for kw in (GROUP_MONO, FAST, dict(walk=glx.WALK_GROUP)):
+        s = glx.Lbvh(G, EPS, 1e-6, **kw)
+        for seed in range(8):
+            pos, _, mass = I.cluster(seed, 300)
+            exact = O.direct_sum(pos, mass, G, EPS)
+            got = s.accelerations(pos, mass)
+            assert I.rms_rel_err(got, exact) < 2e-5, (kw, seed)
+
+
+def test_group_walk_momentum_and_determinism():
+    s = glx.Lbvh(G, EPS, 1e-6, **FAST)
+    pos, _, mass = I.cluster(4, 2000)
+    a = s.accelerations(pos, mass)
+    net = (a * mass[:, None]).sum(axis=0)
+    assert np.linalg.norm(net) / (np.linalg.norm(a, axis=1) * mass).sum() < 1e-4
+    s5 = glx.Lbvh(G, EPS, 0.5, **FAST)
+    assert np.array_equal(s5.accelerations(pos, mass), s5.accelerations(pos, mass))
+
+
+def test_group_walk_empty_single_coincident():
+    s = glx.Lbvh(G, EPS, 0.5, **FAST)
+    assert s.accelerations(np.zeros((0, 3)), np.zeros(0)).shape == (0, 3)
+    assert np.array_equal(s.accelerations(np.array([[1.0, 2.0, 3.0]]), np.ones(1)), np.zeros((1, 3)))
+    pos, mass = I.cluster_with_coincident_knot()
+    exact = O.direct_sum(pos, mass, G, EPS)
+    got = glx.Lbvh(G, EPS, 1e-6, **FAST).accelerations(pos, mass)
+    assert I.rms_rel_err(got, exact) < 2e-5
+
+
+def _errs(a, ref):
+    return I.rel_err_quantiles(a, ref)
+
+
+def test_group_walk_error_not_worse_than_reference_walk():
+    """Plummer N=16384, eps=0.01: per-particle relative error vs FP64 direct sum.
+    Group monopole <= reference per-target walk (same theta); quadrupole at least 3x
+    lower median than group monopole; quadrupole median <= 1e-3 at theta 0.5 and 0.6."""
+    pos, _, mass = I.plummer_numpy(16384, seed=9)
+    eps = 0.01
+    tg = np.arange(16384, dtype=np.uint32)
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, eps, tg)
+    for theta in (0.5, 0.6):
+        ref = glx.Lbvh(1.0, eps, theta).accelerations(pos, mass)
+        gm = glx.Lbvh(1.0, eps, theta, **GROUP_MONO).accelerations(pos, mass)
+        sq = glx.Lbvh(1.0, eps, theta, count=True, **FAST)
+        gq = sq.accelerations(pos, mass)
+        e_ref, e_gm, e_gq = _errs(ref, d64), _errs(gm, d64), _errs(gq, d64)
+        say(f"theta {theta}: ref {e_ref} group-mono {e_gm} group-quad {e_gq} stats {sq.stats()}")
+        assert e_gm[0] <= e_ref[0] and e_gm[1] <= e_ref[1] * 1.05
+        assert e_gq[0] * 3.0 <= e_gm[0]
+        assert e_gq[0] <= 1e-3
+        assert sq.stats()["fallbacks"] == 0
+
+
+def test_fast_mode_collision_error():
+    """Two-galaxy collision input (2^18): FAST mode at theta = 0.6, eps = 0.05; median
+    relative error vs FP64 direct sum on 4096 seeded targets <= 1e-3."""
+    pos, _, mass, _ = ic.collision(1 << 18, 12)
+    rng = np.random.default_rng(2)
+    tg = np.sort(rng.choice(pos.shape[0], 4096, replace=False)).astype(np.uint32)
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, 0.05, tg)
+    s = glx.Lbvh(1.0, 0.05, 0.6, count=True, **FAST)
+    a = s.accelerations(pos, mass)[tg]
+    med, p99 = _errs(a, d64)
+    say("collision 2^18 fast median", med, "p99", p99, "stats", s.stats())
+    assert med <= 1e-3 and p99 <= 2e-2
+    assert s.stats()["fallbacks"] == 0
+
+
+def test_fast_resident_roundtrip_determinism_and_exact_kdk():
+    """FAST resident stepping: bit-identical to upload/step/snapshot round trips and
+    run to run; the velocity change over 20 steps matches the oracle's exact (direct
+    sum) KDK to 5e-3 of its size."""
+    pos, vel, mass = I.cluster(3, 2048, with_vel=True)
+    outs = []
+    for _ in range(2):
+        r = glx.GpuResidentLeapfrog(G, EPS, 0.5, **FAST)
+        r.upload(pos, vel, mass)
+        r.step_many(1e-3, 20)
+        outs.append(r.snapshot())
+    assert np.array_equal(outs[0].pos, outs[1].pos) and np.array_equal(outs[0].vel, outs[1].vel)
+    rt = glx.GpuResidentLeapfrog(G, EPS, 0.5, **FAST)
+    p, v = pos, vel
+    for _ in range(20):
+        rt.upload(p, v, mass)
+        rt.step(1e-3)
+        st = rt.snapshot()
+        p, v = st.pos, st.vel
+    assert np.array_equal(p, outs[0].pos) and np.array_equal(v, outs[0].vel)
+    pe, ve, _ = O.kdk_run(pos, vel, mass, G, EPS, 0.5, 1e-3, 20, solver="direct")
+    dv = np.linalg.norm(outs[0].vel - ve) / np.linalg.norm(ve - vel)
+    say("fast resident dv vs exact KDK (relative to dv_total)", dv)
+    assert dv < 5e-3
