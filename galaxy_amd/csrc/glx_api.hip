@@ -1,128 +1,13 @@
 // C ABI of the LBVH gravity step (include/galaxy_lbvh.h). Every entry point converts
 // runtime failures into a GLX_* status plus a thread-local message.
-#include <math.h>
-#include <string.h>
-
-#include <new>
-#include <string>
-#include <vector>
-
-#include "../../include/galaxy_lbvh.h"
-#include "glx_runtime.h"
+#include "glx_abi_util.h"
 
 using namespace glx;
+using namespace glx::abi;
 
-static thread_local std::string g_last_error;
+thread_local std::string glx::abi::last_error;
 
-template <class F>
-static int guard(F&& f) {
-  try {
-    f();
-    g_last_error.clear();
-    return GLX_OK;
-  } catch (const Error& e) {
-    g_last_error = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return GLX_ENOMEM;
-  } catch (...) {
-    g_last_error = "unexpected internal error";
-    return GLX_EDEVICE;
-  }
-}
-
-extern "C" const char* glx_last_error(void) { return g_last_error.c_str(); }
-
-static constexpr size_t kMaxN = (size_t)1 << 29;
-
-static void validate_params(const glx_params* p) {
-  if (!p) fail(GLX_EINVAL, "params is NULL");
-  if (!isfinite(p->g)) fail(GLX_EINVAL, "g must be finite");
-  if (!isfinite(p->softening) || p->softening < 0.0) fail(GLX_EINVAL, "softening must be >= 0");
-  if (!isfinite(p->theta) || !(p->theta > 0.0)) fail(GLX_EINVAL, "theta must be > 0");
-  if (p->leaf_size > 1) fail(GLX_EINVAL, "leaf_size must be 1");
-  if (p->multipole != 0 && p->multipole != 2) fail(GLX_EINVAL, "multipole must be 0 or 2");
-  if (p->precision > GLX_PREC_KAHAN) fail(GLX_EINVAL, "unknown precision mode");
-  if (p->morton_bits != 0 && p->morton_bits != 30 && p->morton_bits != 63)
-    fail(GLX_EINVAL, "morton_bits must be 30 or 63");
-  if (p->walk > GLX_WALK_GROUP) fail(GLX_EINVAL, "unknown walk");
-  if (p->multipole == 2 && p->walk != GLX_WALK_GROUP)
-    fail(GLX_EINVAL, "quadrupole moments need the group walk");
-}
-
-static Mode mode_from(const glx_params& p) {
-  Mode m;
-  m.key63 = p.morton_bits == 63;
-  m.group = p.walk == GLX_WALK_GROUP;
-  m.quad = p.multipole == 2;
-  m.kahan = p.precision == GLX_PREC_KAHAN;
-  return m;
-}
-
-static void validate_n(size_t n) {
-  if (n > kMaxN) fail(GLX_EINVAL, "n exceeds 2^29 particles per system");
-}
-
-static void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
-  d.n = n;
-  d.g = (float)p.g;
-  d.eps2 = (float)(p.softening * p.softening);
-  d.theta = (float)p.theta;
-  d.precision = p.precision;
-}
-
-static hipStream_t make_stream() {
-  hipStream_t st = nullptr;
-  check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
-  return st;
-}
-
-namespace {
-// RAII device buffer for one-shot calls.
-template <class T>
-struct DevArray {
-  T* p = nullptr;
-  explicit DevArray(size_t n) {
-    if (n) check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
-  }
-  ~DevArray() { (void)hipFree(p); }
-  DevArray(const DevArray&) = delete;
-  DevArray& operator=(const DevArray&) = delete;
-};
-
-struct Stream {
-  hipStream_t st;
-  Stream() : st(make_stream()) {}
-  ~Stream() { (void)hipStreamDestroy(st); }
-};
-
-void h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "upload");
-}
-void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "download");
-}
-void sync(hipStream_t st) { check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
-
-// Zero the walk statistics (not the bottom-up root counter, which the kernels keep
-// at zero between builds).
-void zero_walk_counters(const SegDesc& d, hipStream_t st) {
-  check(hipMemsetAsync(d.counters + kCntInteractions, 0, 2 * sizeof(unsigned long long), st),
-        "memset counters");
-  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 2 * sizeof(unsigned long long), st),
-        "memset counters");
-}
-
-glx_stats stats_from(const unsigned long long* c) {
-  glx_stats s{};
-  s.point_terms = c[kCntInteractions];
-  s.quad_terms = c[kCntQuad];
-  s.node_tests = c[kCntVisits];
-  s.fallbacks = c[kCntOverflow];
-  return s;
-}
-}  // namespace
+extern "C" const char* glx_last_error(void) { return last_error.c_str(); }
 
 extern "C" int glx_device_info(char* name, size_t name_len, int* compute_units,
                                uint64_t* hbm_bytes) {
@@ -465,7 +350,7 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
   });
 }
 
-
+// ----------------------------------------------------------------------------------
 // resident stepper
 // ----------------------------------------------------------------------------------
 
@@ -477,16 +362,9 @@ struct glx_sys {
   uint32_t n = 0;
   double time = 0.0;
   double cur_dt = NAN;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-  }
+  StepGraph graph;
   ~glx_sys() {
-    drop_graph();
+    graph.drop();
     delete batch;
     if (st) (void)hipStreamDestroy(st);
   }
@@ -521,7 +399,7 @@ extern "C" int glx_sys_upload(glx_sys* s, size_t n, const double* pos, const dou
   return guard([&] {
     if (!s) fail(GLX_EINVAL, "sys is NULL");
     validate_n(n);
-    s->drop_graph();
+    s->graph.drop();
     s->time = 0.0;
     s->n = 0;
     s->cur_dt = NAN;
@@ -532,11 +410,7 @@ extern "C" int glx_sys_upload(glx_sys* s, size_t n, const double* pos, const dou
     s->seg.desc.dt = 0.0;
     s->seg.desc.half_dt = 0.0f;
     s->batch->set_segments({s->seg.desc});
-    std::vector<float> v32(3 * n);
-    for (size_t k = 0; k < 3 * n; ++k) v32[k] = (float)vel[k];
-    h2d(s->seg.desc.x, pos, 3 * n * sizeof(double), s->st);
-    h2d(s->seg.desc.v, v32.data(), 3 * n * sizeof(float), s->st);
-    h2d(s->seg.desc.m, mass, n * sizeof(double), s->st);
+    upload_state(s->seg.desc, n, pos, vel, mass, s->st);
     Batch& b = *s->batch;
     b.reset_bbox();
     b.bbox();
@@ -559,30 +433,14 @@ extern "C" int glx_sys_step(glx_sys* s, double dt, uint64_t k) {
       s->batch->set_dt(dt);
       s->cur_dt = dt;
     }
-    if (!s->exec) {
-      check(hipStreamBeginCapture(s->st, hipStreamCaptureModeRelaxed), "BeginCapture");
-      try {
-        s->batch->kick_drift_bbox();
-        s->batch->build_and_walk(1, false);
-      } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(s->st, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      check(hipStreamEndCapture(s->st, &s->graph), "EndCapture");
-      check(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0), "GraphInstantiate");
-    }
+    if (!s->graph.exec) s->graph.capture(*s->batch);
     for (uint64_t q = 0; q < k; ++q) {
-      check(hipGraphLaunch(s->exec, s->st), "GraphLaunch");
+      s->graph.launch(s->st);
       s->time += dt;
     }
     sync(s->st);
   });
 }
-
-// Phases of one resident KDK step, in launch order.
-enum { kPhKick = 0, kPhEncode, kPhSort, kPhGather, kPhKarras, kPhAggregate, kPhWalk, kPhases };
 
 extern "C" int glx_sys_step_timed(glx_sys* s, double dt, uint64_t k, double* phase_ms) {
   return guard([&] {
@@ -597,45 +455,8 @@ extern "C" int glx_sys_step_timed(glx_sys* s, double dt, uint64_t k, double* pha
       s->batch->set_dt(dt);
       s->cur_dt = dt;
     }
-    const size_t ne = (size_t)k * (kPhases + 1);
-    std::vector<hipEvent_t> ev(ne, nullptr);
-    struct EvGuard {
-      std::vector<hipEvent_t>& e;
-      ~EvGuard() {
-        for (auto x : e)
-          if (x) (void)hipEventDestroy(x);
-      }
-    } eg{ev};
-    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
-    Batch& b = *s->batch;
-    for (uint64_t q = 0; q < k; ++q) {
-      hipEvent_t* e = &ev[q * (kPhases + 1)];
-      check(hipEventRecord(e[0], s->st), "hipEventRecord");
-      b.kick_drift_bbox();
-      check(hipEventRecord(e[1], s->st), "hipEventRecord");
-      b.encode();
-      check(hipEventRecord(e[2], s->st), "hipEventRecord");
-      b.sort();
-      check(hipEventRecord(e[3], s->st), "hipEventRecord");
-      b.gather();
-      check(hipEventRecord(e[4], s->st), "hipEventRecord");
-      b.karras();
-      check(hipEventRecord(e[5], s->st), "hipEventRecord");
-      b.aggregate();
-      check(hipEventRecord(e[6], s->st), "hipEventRecord");
-      b.walk(1, false);
-      check(hipEventRecord(e[7], s->st), "hipEventRecord");
-      s->time += dt;
-    }
-    sync(s->st);
-    for (uint64_t q = 0; q < k; ++q) {
-      hipEvent_t* e = &ev[q * (kPhases + 1)];
-      for (int ph = 0; ph < kPhases; ++ph) {
-        float ms = 0.f;
-        check(hipEventElapsedTime(&ms, e[ph], e[ph + 1]), "hipEventElapsedTime");
-        phase_ms[ph] += ms;
-      }
-    }
+    timed_steps(*s->batch, k, phase_ms);
+    s->time += (double)k * dt;
   });
 }
 

@@ -193,6 +193,63 @@ int glx_sys_len(glx_sys *sys, size_t *n);
 /* FP64 device diagnostics; with_potential adds the exact O(n^2) FP64 potential. */
 int glx_sys_diag(glx_sys *sys, int with_potential, glx_diag *out);
 
+/* ---------------------------------------------------------------------------------
+ * Batched and served forms. No reference counterpart: the reference steps one system
+ * per GpuResidentLeapfrog (gpu/src/gpu_resident.rs:1848, up to 64 steps per submit).
+ * A pool keeps many systems ("jobs") resident and advances every active job one KDK
+ * step per glx_pool_step with one hipGraph launch of segment-aware kernels. Jobs are
+ * admitted by a device-memory budget and a slot count and join/leave between steps
+ * (continuous batching). A job's results do not depend on which other jobs share the
+ * launch (every kernel works segment-locally).
+ * ------------------------------------------------------------------------------- */
+typedef struct glx_pool glx_pool;
+
+typedef struct glx_job {
+  size_t n;              /* particles (> 0) */
+  const double *pos;     /* 3n */
+  const double *vel;     /* 3n */
+  const double *mass;    /* n */
+  double g, softening, theta, dt;
+  uint64_t steps;        /* KDK steps to run; 0 = run until retired */
+} glx_job;
+
+typedef struct glx_job_info {
+  uint64_t id;
+  size_t n;
+  uint64_t steps_done, steps;
+  double time;
+  int done;              /* steps_done reached steps: no longer advanced */
+} glx_job_info;
+
+typedef struct glx_pool_info {
+  uint32_t jobs, active;
+  uint64_t particles;    /* in active jobs */
+  uint64_t bytes_used, budget;
+  uint64_t steps;        /* pool steps taken */
+} glx_pool_info;
+
+/* build: walk/multipole/morton_bits/precision shared by every job (g, softening and
+ * theta are taken per job). */
+int glx_pool_create(const glx_params *build, uint64_t budget_bytes, uint32_t max_jobs,
+                    glx_pool **out);
+int glx_pool_destroy(glx_pool *pool);
+/* Uploads the job and primes a(x0). GLX_EBUDGET / GLX_ENOSLOT when it does not fit. */
+int glx_pool_admit(glx_pool *pool, const glx_job *job, uint64_t *id);
+/* One KDK step of every active job; device_ms (may be NULL) = device time of the step. */
+int glx_pool_step(glx_pool *pool, double *device_ms);
+/* One step with direct launches and per-phase device ms (see glx_sys_step_timed). */
+int glx_pool_step_timed(glx_pool *pool, double *phase_ms);
+/* Walk statistics summed over active jobs at their current positions. */
+int glx_pool_count(glx_pool *pool, glx_stats *out);
+int glx_pool_job(glx_pool *pool, uint64_t id, glx_job_info *out);
+int glx_pool_snapshot(glx_pool *pool, uint64_t id, double *pos, double *vel);
+int glx_pool_snapshot_accel(glx_pool *pool, uint64_t id, double *acc);
+/* Copies the job out (pos/vel may be NULL) and frees its device memory. */
+int glx_pool_retire(glx_pool *pool, uint64_t id, double *pos, double *vel);
+int glx_pool_status(glx_pool *pool, glx_pool_info *out);
+/* Device bytes a job of n particles takes against the budget (0 if n is invalid). */
+uint64_t glx_pool_job_bytes(size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,13 +13,17 @@ import galaxy_amd
 from galaxy_amd import lbvh as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "galaxy_lbvh.h")
+INCLUDE = os.path.join(ROOT, "include")
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(glx_\w+)\s*\(", text)))
+    names = set()
+    for h in sorted(os.listdir(INCLUDE)):
+        if h.endswith(".h"):
+            text = re.sub(r"/\*.*?\*/", "", open(os.path.join(INCLUDE, h)).read(), flags=re.S)
+            text = re.sub(r"//[^\n]*", "", text)
+            names |= set(re.findall(r"\b(glx_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
@@ -36,8 +40,26 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_binding_covers_header():
+    from galaxy_amd import ic, pool
+    pool._bind()
+    ic._bind()
     for n in declared_functions():
         assert getattr(L.lib(), n).argtypes is not None or n == "glx_last_error", n
+
+
+def test_pool_arguments_rejected_before_device():
+    from galaxy_amd import pool
+    lib = pool._bind()
+    h = C.c_void_p()
+    p = L.Params(1.0, 0.0, 1.0, 1, 0, 0, 0)
+    assert lib.glx_pool_create(C.byref(p), 1 << 30, 0, C.byref(h)) == L.GLX_EINVAL
+    assert lib.glx_pool_create(None, 1 << 30, 4, C.byref(h)) == L.GLX_EINVAL
+    bad = L.Params(1.0, 0.0, 1.0, 1, 0, 0, 0, 31, 0)  # morton_bits
+    assert lib.glx_pool_create(C.byref(bad), 1 << 30, 4, C.byref(h)) == L.GLX_EINVAL
+    assert lib.glx_pool_step(None, None) == L.GLX_EINVAL
+    assert lib.glx_pool_job_bytes(0) == 0
+    assert lib.glx_pool_job_bytes(1000) > 1000 * 64
+    assert lib.glx_pool_job_bytes(2000) > lib.glx_pool_job_bytes(1000)
 
 
 def test_invalid_params_rejected_before_device():

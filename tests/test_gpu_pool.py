@@ -1,0 +1,139 @@
+"""GPU tests of the batched/served pool (glx_pool_*).
+
+Every kernel works segment-locally (per-system bbox, keys, sort offsets, tree and walk),
+so a job's trajectory must be bit-identical whether it runs alone, in a batch of uneven
+systems, or while other jobs are admitted and retired around it; and a one-job pool must
+reproduce the single-system resident stepper bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import galaxy_amd as glx
+from galaxy_amd import ic
+from galaxy_amd.pool import Pool
+from oracle import oracle as O
+from tests import inputs as I
+
+pytestmark = pytest.mark.gpu
+GB = 1 << 30
+MODES = {
+    "reference": dict(morton_bits=30, walk=glx.WALK_REFERENCE, multipole=0),
+    "fast": dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2),
+}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def systems(sizes, seed=0):
+    out = []
+    for k, n in enumerate(sizes):
+        pos, vel, mass = ic.plummer(int(n), seed + k)
+        out.append((pos, vel, mass))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["reference", "fast"])
+def test_one_job_pool_equals_resident_stepper(mode):
+    pos, vel, mass = ic.plummer(3000, 5)
+    p = Pool(GB, **MODES[mode])
+    j = p.admit(pos, vel, mass, softening=0.01, theta=0.5, dt=1e-3)
+    for _ in range(7):
+        p.step()
+    pp, pv = p.snapshot(j)
+    r = glx.GpuResidentLeapfrog(1.0, 0.01, 0.5, **MODES[mode])
+    r.upload(pos, vel, mass)
+    r.step_many(1e-3, 7)
+    s = r.snapshot()
+    assert np.array_equal(pp, s.pos) and np.array_equal(pv, s.vel)
+
+
+@pytest.mark.parametrize("mode", ["reference", "fast"])
+def test_batch_composition_invariance(mode):
+    sizes = [1024, 1, 2, 5000, 777, 2049, 3000, 64]
+    sy = systems(sizes, seed=10)
+    thetas = [0.4, 0.5, 0.7, 0.5, 0.6, 0.5, 0.45, 0.5]
+    # alone
+    alone = []
+    for (pos, vel, mass), th in zip(sy, thetas):
+        p = Pool(GB, **MODES[mode])
+        j = p.admit(pos, vel, mass, softening=0.01, theta=th, dt=1e-3)
+        for _ in range(5):
+            p.step()
+        alone.append(p.snapshot(j))
+    # all together
+    p = Pool(GB, **MODES[mode])
+    ids = [p.admit(pos, vel, mass, softening=0.01, theta=th, dt=1e-3)
+           for (pos, vel, mass), th in zip(sy, thetas)]
+    for _ in range(5):
+        p.step()
+    for k, j in enumerate(ids):
+        pp, pv = p.snapshot(j)
+        assert np.array_equal(pp, alone[k][0]) and np.array_equal(pv, alone[k][1]), (mode, k)
+
+
+def test_continuous_batching_admit_retire_mid_run():
+    sy = systems([4000, 1500, 2500, 800], seed=30)
+    ref = Pool(GB)
+    jr = ref.admit(*sy[0], softening=0.01, theta=0.5, dt=1e-3)
+    for _ in range(12):
+        ref.step()
+    want = ref.snapshot(jr)
+
+    p = Pool(GB)
+    j0 = p.admit(*sy[0], softening=0.01, theta=0.5, dt=1e-3)
+    for _ in range(3):
+        p.step()
+    j1 = p.admit(*sy[1], softening=0.05, theta=0.7, dt=2e-3, steps=4)
+    for _ in range(2):
+        p.step()
+    j2 = p.admit(*sy[2], softening=0.01, theta=0.4, dt=1e-3)
+    for _ in range(3):
+        p.step()
+    p.retire(j2)
+    j3 = p.admit(*sy[3], softening=0.02, theta=0.5, dt=1e-3)
+    for _ in range(4):
+        p.step()
+    got = p.snapshot(j0)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    info = p.job(j1)
+    assert info["done"] and info["steps_done"] == 4 and abs(info["time"] - 8e-3) < 1e-15
+    assert p.job(j0)["steps_done"] == 12 and p.job(j3)["steps_done"] == 4
+    st = p.status()
+    assert st["jobs"] == 3 and st["active"] == 2
+
+
+def test_admission_budget_and_slots():
+    pos, vel, mass = ic.plummer(5000, 1)
+    need = Pool.job_bytes(5000)
+    p = Pool(int(2.5 * need), max_jobs=8)
+    assert p.admit(pos, vel, mass) is not None
+    assert p.admit(pos, vel, mass) is not None
+    assert p.admit(pos, vel, mass) is None  # budget
+    st = p.status()
+    assert st["bytes_used"] == 2 * need and st["jobs"] == 2
+    q = Pool(100 * need, max_jobs=2)
+    assert q.admit(pos, vel, mass) is not None and q.admit(pos, vel, mass) is not None
+    assert q.admit(pos, vel, mass) is None  # slots
+    jid = [k for k in q._n][0]
+    q.retire(jid)
+    assert q.admit(pos, vel, mass) is not None  # slot freed
+    with pytest.raises(glx.GlxError):
+        q.job(10_000)
+
+
+def test_pool_forces_match_oracle_per_job():
+    sy = systems([2000, 333, 4096], seed=50)
+    p = Pool(GB)
+    ids = [p.admit(pos, vel, mass, softening=0.01, theta=0.5, dt=1e-3) for pos, vel, mass in sy]
+    for j, (pos, vel, mass) in zip(ids, sy):
+        a = p.snapshot_accel(j)
+        cpu, _ = O.lbvh_accelerations(pos, mass, 1.0, 0.01, 0.5)
+        med, p99 = I.rel_err_quantiles(a, cpu)
+        assert med < 1e-5 and p99 < 1e-3, (j, med, p99)
+    cnt = p.count()
+    want = sum(O.lbvh_accelerations(pos, mass, 1.0, 0.01, 0.5)[1] for pos, vel, mass in sy)
+    assert abs(cnt["point_terms"] - want) / want < 1e-3
