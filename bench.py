@@ -1,22 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the LBVH gravity step on one MI355X.
 
-Workload (the configuration the headline metric is quoted on that fits one GPU):
-two-galaxy collision, N = 2^20 f32-representable particles, eps = 0.05, theta = 0.6,
-dt = 5e-3, G = 1, through the plain single-system resident path. One step = one KDK
-step with a full rebuild: kick-drift+bbox, Morton encode, radix sort, gather, Karras
-build, bottom-up aggregate, walk + closing kick.
+Headline workload (the mixed-size ensemble the served-vs-plain metric is quoted on):
+512 independent Plummer spheres (G = M = a = 1, truncated at r = 10), sizes
+N_s = floor(exp(U(ln 1024, ln 65536))) from a seed (about 7.9 M particles in all),
+eps = 0.01, theta = 0.5, dt = 1e-3. Every system is resident in one pool and all are
+advanced together by one hipGraph launch per KDK step (segment-aware kernels:
+kick-drift + per-system bbox, 63-bit Morton encode, segmented radix sort, gather,
+Karras build, bottom-up monopole + quadrupole aggregate, group walk + closing kick).
 
-Two builds of the step are measured on the same input:
-  throughput  63-bit Morton keys, group walk, quadrupole moments (the headline `value`)
-  reference   30-bit keys, per-target walk, monopole: the reference's algorithm, whose
-              codes/sort/topology are bit-exact with the reference CPU path
-Each reports its force error against an FP64 direct sum on 4096 seeded targets.
+  value            aggregate particle-forces/s of the pool = sum(N_s) * K / wall(K steps)
+  plain            the same systems with the same build, one system at a time through
+                   the single-system resident path (glx_sys_*)
+  p50/p99 step     per-step device time of the pool step (HIP events on its stream)
+  force error      |a - a_fp64| / |a_fp64| on seeded targets of every system, against an
+                   FP64 direct sum at the same positions
 
-Prints ONE JSON line. `value` = particle-forces/s = N * K / (wall time of K steps), inputs
-resident in HBM, bracketed by device synchronisation. The walk's roofline counts
+Also measured: the two-galaxy collision, N = 2^20, eps = 0.05, theta = 0.6, dt = 5e-3,
+through the plain single-system path, in the fast build and in the reference build
+(30-bit keys, per-target walk, monopole: codes, order and topology bit-exact with the
+reference CPU path).
+
+Roofline: the walk (k_walk_group) dominates; it is FP32-VALU bound and counted at
 20 flops per point-mass term and 56 per quadrupole term (terms counted by the kernel on
-the same input) over the walk's average device time from HIP events on its stream.
+the same positions) over its average device time from HIP events on the pool's stream.
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -33,6 +41,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-forces/sec at 1e-3 median force err, served vs plain; p50/p99 step"
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector with packed FP32; 78.6 without
+FP32_NONPACKED_TFLOPS = 78.6
 FLOPS_POINT = 20
 FLOPS_QUAD = 56
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
@@ -42,10 +51,73 @@ PHASE_BYTES_63 = {"kick_drift_bbox": 72, "encode": 36, "sort": 196, "gather": 36
                   "aggregate": 264}
 
 
-def cpu_baseline(pos, mass, eps, theta, gpu_acc, budget_s=12.0, seed=7):
-    """Time the oracle (C restatement of the reference's f64 LBVH, 1 thread) on a
-    bounded sample: one full build plus the walk of S seeded targets, extrapolated to
-    one full force evaluation. Also returns the reference-mode GPU error against it."""
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def ensemble_sizes(seed: int, count: int = 512, lo: int = 1024, hi: int = 65536):
+    rng = np.random.default_rng(seed)
+    n = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+    return np.clip(n, lo, hi)
+
+
+def quantiles(e):
+    return {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
+            "targets": int(np.size(e))}
+
+
+def rel_err(a, ref):
+    return np.linalg.norm(a - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+
+
+# ------------------------------------------------------------------ CPU baseline
+
+
+def cpu_ensemble(systems, eps, theta, budget_s, seed=3):
+    """The oracle (C restatement of solvers/src/lbvh.rs, f64) on whole systems of the
+    ensemble drawn in seeded random order until `budget_s` is spent: full build + walk of
+    every target per system; particle-forces/s = sum N / sum t. One thread (the
+    reference's accelerations loop is serial), then all cores on a few of the same
+    systems."""
+    from oracle import oracle as O
+
+    O.lib()
+    order = np.random.default_rng(seed).permutation(len(systems))
+    n_done, t_done, used = 0, 0.0, []
+    for k in order:
+        pos, _, mass = systems[k]
+        t0 = time.perf_counter()
+        O.lbvh_accelerations(pos, mass, 1.0, eps, theta)
+        t_done += time.perf_counter() - t0
+        n_done += mass.size
+        used.append(int(k))
+        if t_done > budget_s:
+            break
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    n_mt, t_mt = 0, 0.0
+    for k in used[: max(1, len(used) // 2)]:
+        pos, _, mass = systems[k]
+        t0 = time.perf_counter()
+        O.lbvh_accelerations(pos, mass, 1.0, eps, theta, nthreads=cores)
+        t_mt += time.perf_counter() - t0
+        n_mt += mass.size
+    return {
+        "value": n_done / t_done,
+        "unit": "particle-forces/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (C restatement of solvers/src/lbvh.rs, f64, -O3, 1 thread): full "
+                   f"build + walk of every target for {len(used)} of the 512 systems drawn in "
+                   f"seeded order ({n_done} particles, {t_done:.1f} s)"),
+        "all_cores": {"value": n_mt / t_mt, "cores": cores,
+                      "sample": f"first {max(1, len(used) // 2)} of the same systems"},
+    }
+
+
+def cpu_collision(pos, mass, eps, theta, gpu_acc, budget_s, seed=7):
+    """The oracle on the 2^20 collision: one full build plus the walk of S seeded
+    targets (1 thread), extrapolated to one full force evaluation; also the error of the
+    reference-build GPU accelerations against it on those targets."""
     from oracle import oracle as O
 
     n = mass.size
@@ -64,24 +136,83 @@ def cpu_baseline(pos, mass, eps, theta, gpu_acc, budget_s=12.0, seed=7):
     acc, cnt = O.lbvh_accelerations(pos, mass, 1.0, eps, theta, targets=tg)
     t_walk = time.perf_counter() - t0 - t_build
     t_full = t_build + n * t_walk / s
-    e = np.linalg.norm(gpu_acc[tg] - acc, axis=1) / np.maximum(np.linalg.norm(acc, axis=1), 1e-300)
     return {
-        "value": n / t_full,
-        "unit": "particle-forces/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"oracle (C restatement of solvers/src/lbvh.rs, -O3, 1 thread) on the same "
-                   f"positions: full build {t_build:.2f}s + walk of {s} seeded targets "
-                   f"{t_walk:.2f}s, extrapolated to all {n} targets = {t_full:.1f}s per force "
-                   f"evaluation"),
-        "steps_per_s": 1.0 / t_full,
+        "value": n / t_full, "unit": "particle-forces/s", "cores": 1, "kind": "port",
+        "sample": (f"full build {t_build:.2f}s + walk of {s} seeded targets {t_walk:.2f}s, "
+                   f"extrapolated to all {n} targets = {t_full:.1f}s per force evaluation"),
         "interactions_per_particle": cnt / s,
-        "reference_mode_gpu_vs_oracle": {"median": float(np.median(e)),
-                                         "p99": float(np.quantile(e, 0.99)), "targets": s},
+        "reference_build_gpu_vs_oracle": quantiles(rel_err(gpu_acc[tg], acc)),
     }
 
 
-def run_mode(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets):
+# ------------------------------------------------------------------ GPU runs
+
+
+def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_system):
+    n_tot = int(sum(m.size for _, _, m in systems))
+    need = sum(Pool.job_bytes(m.size) for _, _, m in systems)
+    pool = Pool(need + (1 << 20), max_jobs=len(systems), **mode)
+    t0 = time.perf_counter()
+    ids = [pool.admit(p, v, m, softening=eps, theta=theta, dt=dt) for p, v, m in systems]
+    t_admit = time.perf_counter() - t0
+    assert all(j is not None for j in ids)
+    for _ in range(max(1, warmup)):  # the first step captures the graph
+        pool.step()
+    cnt = pool.count()
+    timed = [pool.step_timed() for _ in range(3)]
+    phases = {k: float(np.mean([t[k] for t in timed])) for k in timed[0]}
+    dev = np.zeros(steps)
+    t0 = time.perf_counter()
+    for q in range(steps):
+        dev[q] = pool.step()
+    wall = time.perf_counter() - t0
+    rng = np.random.default_rng(17)
+    errs = []
+    for j, (_, _, m) in zip(ids, systems):
+        pos, _ = pool.snapshot(j)
+        acc = pool.snapshot_accel(j)
+        tg = np.sort(rng.choice(m.size, min(m.size, err_per_system), replace=False))
+        errs.append(rel_err(acc[tg], glx.direct_sum_f64(pos, m, 1.0, eps, tg.astype(np.uint32))))
+    walk_ms = phases["walk"]
+    flops = FLOPS_POINT * cnt["point_terms"] + FLOPS_QUAD * cnt["quad_terms"]
+    pool.close()
+    return {
+        "value": n_tot * steps / wall,
+        "ms_per_step": wall * 1e3 / steps,
+        "device_ms": {"p50": float(np.percentile(dev, 50)), "p99": float(np.percentile(dev, 99)),
+                      "mean": float(dev.mean())},
+        "phase_ms_per_step": phases,
+        "phase_hbm_gbs": {k: PHASE_BYTES_63[k] * n_tot / (phases[k] * 1e-3) / 1e9
+                          for k in PHASE_BYTES_63},
+        "walk": {"ms": walk_ms, "flops": flops, "tflops": flops / (walk_ms * 1e-3) / 1e12,
+                 "point_terms_per_particle": cnt["point_terms"] / n_tot,
+                 "quad_terms_per_particle": cnt["quad_terms"] / n_tot,
+                 "node_tests_per_particle": cnt["node_tests"] / n_tot,
+                 "fallbacks": cnt["fallbacks"]},
+        "error": quantiles(np.concatenate(errs)),
+        "admit_s": t_admit,
+        "particles": n_tot,
+    }
+
+
+def plain(glx, systems, eps, theta, dt, steps, mode):
+    n_f, t_f, t_all = 0, 0.0, 0.0
+    t0a = time.perf_counter()
+    for pos, vel, mass in systems:
+        r = glx.GpuResidentLeapfrog(1.0, eps, theta, **mode)
+        r.upload(pos, vel, mass)
+        r.step_many(dt, 1)  # captures the graph
+        t0 = time.perf_counter()
+        r.step_many(dt, steps)
+        t_f += time.perf_counter() - t0
+        n_f += mass.size * steps
+        r.close()
+    t_all = time.perf_counter() - t0a
+    return {"value": n_f / t_f, "ms_per_ensemble_step": t_f * 1e3 / steps,
+            "steps_per_system": steps, "wall_incl_upload_s": t_all}
+
+
+def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets):
     n = mass.size
     sysm = glx.GpuResidentLeapfrog(1.0, eps, theta, **mode)
     sysm.upload(pos, vel, mass)
@@ -90,119 +221,295 @@ def run_mode(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targe
     snap = sysm.snapshot()
     acc0 = sysm.snapshot_accel()
     c0 = sysm.count()
-    sysm.snapshot()  # synchronises the stream
     t0 = time.perf_counter()
     phases = sysm.step_timed(dt, steps)
     wall = time.perf_counter() - t0
-    c1 = sysm.count()
     g0 = time.perf_counter()
     sysm.step_many(dt, steps)
     graph_ms = (time.perf_counter() - g0) * 1e3 / steps
     d64 = glx.direct_sum_f64(snap.pos, mass, 1.0, eps, d64_targets)
-    e = np.linalg.norm(acc0[d64_targets] - d64, axis=1) / np.linalg.norm(d64, axis=1)
-    pt = 0.5 * (c0["point_terms"] + c1["point_terms"])
-    qt = 0.5 * (c0["quad_terms"] + c1["quad_terms"])
     walk_ms = phases["walk"] / steps
-    flops = FLOPS_POINT * pt + FLOPS_QUAD * qt
-    achieved = flops / (walk_ms * 1e-3) / 1e12
+    flops = FLOPS_POINT * c0["point_terms"] + FLOPS_QUAD * c0["quad_terms"]
     bytes_tab = PHASE_BYTES_63 if mode.get("morton_bits") == 63 else PHASE_BYTES_30
     if mode.get("multipole") != 2:
         bytes_tab = dict(bytes_tab, aggregate=168)
+    sysm.close()
     return {
         "snap": snap, "acc0": acc0,
         "value": n * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
         "graph_ms_per_step": graph_ms,
         "phase_ms_per_step": {k: v / steps for k, v in phases.items()},
-        "phase_hbm_gbs": {k: bytes_tab[k] * n / (phases[k] / steps * 1e-3) / 1e9 for k in bytes_tab},
-        "walk": {"ms": walk_ms, "tflops": achieved, "flops": flops,
-                 "point_terms_per_particle": pt / n, "quad_terms_per_particle": qt / n,
-                 "node_tests_per_particle": 0.5 * (c0["node_tests"] + c1["node_tests"]) / n,
-                 "fallbacks": c0["fallbacks"] + c1["fallbacks"]},
-        "error": {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
-                  "targets": int(d64_targets.size)},
+        "phase_hbm_gbs": {k: bytes_tab[k] * n / (phases[k] / steps * 1e-3) / 1e9
+                          for k in bytes_tab},
+        "walk": {"ms": walk_ms, "tflops": flops / (walk_ms * 1e-3) / 1e12, "flops": flops,
+                 "point_terms_per_particle": c0["point_terms"] / n,
+                 "quad_terms_per_particle": c0["quad_terms"] / n,
+                 "node_tests_per_particle": c0["node_tests"] / n,
+                 "fallbacks": c0["fallbacks"]},
+        "error": quantiles(rel_err(acc0[d64_targets], d64)),
     }
+
+
+def precision(glx, ic, n, seed, steps, sample=65536, eps=0.02, theta=0.5, dt=2e-3,
+              diag_every=0):
+    """Precision stress: 1:3 two-galaxy collision. The FP32 walk with Kahan-compensated
+    accumulation against the all-FP32 walk (both with FP64 drift and FP64 diagnostics):
+    force error against an FP64 direct sum on `sample` seeded particles, then energy and
+    momentum drift over `steps` KDK steps (exact O(N^2) FP64 potential at the ends)."""
+    pos, vel, mass, _ = ic.collision(n, seed, mass_ratio=3.0)
+    tg = np.sort(np.random.default_rng(5).choice(n, min(n, sample), replace=False))
+    tg = tg.astype(np.uint32)
+    t0 = time.perf_counter()
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, eps, tg)
+    log(f"precision: fp64 direct sum on {tg.size} targets {time.perf_counter() - t0:.1f}s")
+    res = {"n": n, "mass_ratio": 3.0, "eps": eps, "theta": theta, "dt": dt, "steps": steps,
+           "sample": int(tg.size)}
+    for name, prec in (("f32", glx.PREC_F32), ("kahan", glx.PREC_KAHAN)):
+        r = glx.GpuResidentLeapfrog(1.0, eps, theta, precision=prec, morton_bits=63,
+                                    walk=glx.WALK_GROUP, multipole=2)
+        r.upload(pos, vel, mass)
+        e = rel_err(r.snapshot_accel()[tg], d64)
+        d0 = r.diag(with_potential=True)
+        e0 = d0["kinetic"] + d0["potential"]
+        psc = float(np.sum(mass * np.linalg.norm(vel, axis=1)))
+        pdrift, trace = 0.0, []
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(diag_every or steps, steps - done)
+            r.step_many(dt, k)
+            done += k
+            d = r.diag(with_potential=False)
+            pdrift = max(pdrift, float(np.linalg.norm(d["momentum"] - d0["momentum"])) / psc)
+            trace.append(done)
+        t_steps = time.perf_counter() - t0
+        d1 = r.diag(with_potential=True)
+        e1 = d1["kinetic"] + d1["potential"]
+        res[name] = {"force_error_vs_fp64_direct": quantiles(e),
+                     "energy_drift": abs(e1 - e0) / abs(e0),
+                     "momentum_drift_max": pdrift,
+                     "ms_per_step": t_steps * 1e3 / max(steps, 1),
+                     "energy": [e0, e1]}
+        log(f"precision {name}: {res[name]}")
+        r.close()
+    return res
+
+
+def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nmax=2097152):
+    """Serving stream: collision jobs arrive as a Poisson process at `rate` per second
+    (N log-uniform in [nmin, nmax], theta uniform in {0.4, 0.5, 0.7}, steps uniform in
+    [50, 500], eps = 0.05, dt = 5e-3); a pool with a device-memory budget admits them in
+    arrival order as memory frees up and advances every active job by one KDK step per
+    iteration; finished jobs are retired. Initial conditions are made by a client thread
+    ahead of their arrival time (outside the server's clock when it keeps up)."""
+    import queue
+    import threading
+
+    rng = np.random.default_rng(seed)
+    njobs = int(rate * seconds * 1.5) + 8
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    sizes = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), njobs))).astype(int)
+    thetas = rng.choice([0.4, 0.5, 0.7], njobs)
+    nsteps = rng.integers(50, 501, njobs)
+    ready: queue.Queue = queue.Queue(maxsize=8)
+    stop = threading.Event()
+
+    def client():
+        for k in range(njobs):
+            if stop.is_set():
+                return
+            pos, vel, mass, _ = ic.collision(int(sizes[k]), seed + 1000 + k)
+            while not stop.is_set():
+                try:
+                    ready.put((k, pos, vel, mass), timeout=0.2)
+                    break
+                except queue.Full:
+                    pass
+
+    th = threading.Thread(target=client, daemon=True)
+    th.start()
+    pool = Pool(budget, max_jobs=4096, morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
+    waiting, active = [], {}
+    lat, pf, wall_steps = [], 0, 0.0
+    admitted = finished = rejected_wait = 0
+    queue_delay = []
+    t_start = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t_start
+        if now >= seconds:
+            break
+        while not ready.empty():
+            waiting.append(ready.get())
+        waiting.sort(key=lambda w: w[0])
+        still = []
+        for w in waiting:
+            k = w[0]
+            if arrivals[k] > now:
+                still.append(w)
+                continue
+            jid = pool.admit(w[1], w[2], w[3], softening=0.05, theta=float(thetas[k]), dt=5e-3,
+                             steps=int(nsteps[k]))
+            if jid is None:
+                rejected_wait += 1
+                still.append(w)
+                continue
+            active[jid] = w[3].size
+            admitted += 1
+            queue_delay.append(time.perf_counter() - t_start - arrivals[k])
+        waiting = still
+        if not active:
+            time.sleep(0.001)
+            continue
+        t0 = time.perf_counter()
+        ms = pool.step()
+        wall_steps += time.perf_counter() - t0
+        lat.append(ms)
+        pf += sum(active.values())
+        for jid in list(active):
+            if pool.job(jid)["done"]:
+                pool.retire(jid)
+                del active[jid]
+                finished += 1
+    elapsed = time.perf_counter() - t_start
+    stop.set()
+    th.join(timeout=30)
+    st = pool.status()
+    pool.close()
+    lat = np.array(lat) if lat else np.zeros(1)
+    return {
+        "seconds": elapsed, "rate_per_s": rate, "budget_bytes": budget,
+        "jobs_admitted": admitted, "jobs_finished": finished, "jobs_active_at_end": st["active"],
+        "steps": int(lat.size),
+        "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+        "particle_forces_per_s": pf / elapsed,
+        "particle_forces_per_s_stepping": pf / max(wall_steps, 1e-9),
+        "queue_delay_s": {"p50": float(np.percentile(queue_delay, 50)) if queue_delay else 0.0,
+                          "p99": float(np.percentile(queue_delay, 99)) if queue_delay else 0.0},
+    }
+
+
+def strip(d):
+    return {k: v for k, v in d.items() if k not in ("snap", "acc0")}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--collision-n", type=int, default=1 << 20)
+    ap.add_argument("--collision-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-reference-mode", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-collision", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--profile", action="store_true", help="short run for profilers")
+    ap.add_argument("--precision-n", type=int, default=0,
+                    help="also run the 1:3 collision precision stress at this N (0: skip)")
+    ap.add_argument("--precision-steps", type=int, default=1000)
+    ap.add_argument("--serve-seconds", type=float, default=0.0,
+                    help="also run the serving stream for this long (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="job arrivals per second")
+    ap.add_argument("--profile", action="store_true",
+                    help="pool steps only (for profilers): no plain, collision or CPU legs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
-                                   "of it (independent systems shard across GPUs with no "
-                                   "communication; see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one GPU; runs on several GPUs are "
+                                   "not part of it (independent systems would shard across "
+                                   "GPUs with no communication)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.profile:
+        args.no_cpu = args.no_plain = args.no_collision = True
 
     import galaxy_amd as glx
     from galaxy_amd import ic
+    from galaxy_amd.pool import Pool
 
-    eps, theta, dt = 0.05, 0.6, 5e-3
-    n = args.n
-    pos, vel, mass, info = ic.collision(n, args.seed)
-    rng = np.random.default_rng(11)
-    tg = np.sort(rng.choice(n, min(n, 4096), replace=False)).astype(np.uint32)
     fast_mode = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
     ref_mode = dict(morton_bits=30, walk=glx.WALK_REFERENCE, multipole=0)
+    eps, theta, dt = 0.01, 0.5, 1e-3
 
-    fast = run_mode(glx, pos, vel, mass, eps, theta, dt, args.steps, args.warmup, fast_mode, tg)
+    sizes = ensemble_sizes(args.seed, args.systems)
+    t0 = time.perf_counter()
+    systems = [ic.plummer(int(n), args.seed + 1 + k) for k, n in enumerate(sizes)]
+    log(f"ensemble: {len(systems)} systems, {int(sizes.sum())} particles, "
+        f"ICs {time.perf_counter() - t0:.1f}s")
+    b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, fast_mode, 8)
+    log(f"batched: {b['value']:.4g} pf/s, {b['ms_per_step']:.2f} ms/step, err {b['error']}")
     out = {
         "metric": METRIC,
-        "value": fast["value"],
+        "value": b["value"],
         "unit": "particle-forces/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": fast["ms_per_step"],
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "two-galaxy collision, plain single-system resident path",
-                   "n": n, "theta": theta, "eps": eps, "dt": dt, "seed": args.seed,
-                   "build": "63-bit Morton, group walk, quadrupole", "leaf_size": 1},
+        "config": {"workload": "batched ensemble of Plummer spheres, one pool, one graph "
+                               "launch per KDK step",
+                   "systems": len(systems), "particles": int(sizes.sum()),
+                   "n_range": [1024, 65536], "theta": theta, "eps": eps, "dt": dt,
+                   "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
+                   "leaf_size": 1},
+        "p50_step_ms": b["device_ms"]["p50"],
+        "p99_step_ms": b["device_ms"]["p99"],
+        "force_error_vs_fp64_direct": b["error"],
         "roofline": {
             "bound": "fp32-valu",
             "kernel": "k_walk_group",
-            "achieved": fast["walk"]["tflops"],
+            "achieved": b["walk"]["tflops"],
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": fast["walk"]["tflops"] / FP32_PEAK_TFLOPS,
-            "frac_vs_non_packed_peak": fast["walk"]["tflops"] / 78.6,
+            "frac": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
+            "frac_vs_non_packed_peak": b["walk"]["tflops"] / FP32_NONPACKED_TFLOPS,
             "traffic": None,
-            "flops_per_launch": fast["walk"]["flops"],
-            "walk_ms": fast["walk"]["ms"],
+            "flops_per_launch": b["walk"]["flops"],
+            "walk_ms": b["walk"]["ms"],
         },
-        "force_error_vs_fp64_direct": fast["error"],
-        "steps_per_s": 1e3 / fast["ms_per_step"],
-        "graph_ms_per_step": fast["graph_ms_per_step"],
-        "phase_ms_per_step": fast["phase_ms_per_step"],
-        "phase_hbm_gbs": fast["phase_hbm_gbs"],
-        "walk_stats": fast["walk"],
-        "device": glx.device_info(),
+        "batched": strip(b),
     }
-    ref = None
-    if not args.no_reference_mode and not args.profile:
-        ref = run_mode(glx, pos, vel, mass, eps, theta, dt, 1, 1, ref_mode, tg)
-        out["reference_mode"] = {
-            "build": "30-bit Morton, per-target walk, monopole (bit-exact codes/sort/topology)",
-            "value": ref["value"], "ms_per_step": ref["ms_per_step"],
-            "phase_ms_per_step": ref["phase_ms_per_step"], "walk_stats": ref["walk"],
-            "force_error_vs_fp64_direct": ref["error"],
+    if not args.no_plain:
+        p = plain(glx, systems, eps, theta, dt, args.plain_steps, fast_mode)
+        log(f"plain: {p['value']:.4g} pf/s")
+        out["plain"] = p
+        out["served_vs_plain"] = b["value"] / p["value"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_ensemble(systems, eps, theta, args.cpu_budget)
+        log(f"cpu: {out['cpu_baseline']['value']:.4g} pf/s")
+    del systems
+    if not args.no_collision:
+        ce, ct, cdt = 0.05, 0.6, 5e-3
+        n = args.collision_n
+        pos, vel, mass, _ = ic.collision(n, args.seed)
+        tg = np.sort(np.random.default_rng(11).choice(n, min(n, 4096), replace=False))
+        tg = tg.astype(np.uint32)
+        f = single(glx, pos, vel, mass, ce, ct, cdt, args.collision_steps, 2, fast_mode, tg)
+        log(f"collision fast: {f['value']:.4g} pf/s")
+        r = single(glx, pos, vel, mass, ce, ct, cdt, 1, 1, ref_mode, tg)
+        log(f"collision reference build: {r['value']:.4g} pf/s")
+        out["collision_2p20"] = {
+            "workload": "two-galaxy collision, plain single-system resident path",
+            "n": n, "theta": ct, "eps": ce, "dt": cdt,
+            "fast": strip(f),
+            "reference_build": strip(r),
         }
-    if not args.no_cpu and not args.profile:
-        snap = (ref or fast)["snap"]
-        acc_ref = ref["acc0"] if ref else fast["acc0"]
-        out["cpu_baseline"] = cpu_baseline(snap.pos, mass, eps, theta, acc_ref, args.cpu_budget)
+        if not args.no_cpu:
+            out["collision_2p20"]["cpu_baseline"] = cpu_collision(
+                r["snap"].pos, mass, ce, ct, r["acc0"], args.cpu_budget)
+    if args.serve_seconds > 0:
+        out["serving"] = serve(glx, ic, Pool, args.serve_seconds, args.serve_rate, args.seed)
+        log(f"serving: {out['serving']}")
+    if args.precision_n:
+        out["precision_stress"] = precision(glx, ic, args.precision_n, args.seed,
+                                            args.precision_steps, diag_every=100)
+    out["device"] = glx.device_info()
     print(json.dumps(out), flush=True)
 
 

@@ -37,13 +37,16 @@ int guard(F&& f) {
 }
 
 constexpr size_t kMaxN = (size_t)1 << 29;
+constexpr uint32_t kMaxLeafSize = 32;
 
 inline void validate_params(const glx_params* p) {
   if (!p) fail(GLX_EINVAL, "params is NULL");
   if (!isfinite(p->g)) fail(GLX_EINVAL, "g must be finite");
   if (!isfinite(p->softening) || p->softening < 0.0) fail(GLX_EINVAL, "softening must be >= 0");
   if (!isfinite(p->theta) || !(p->theta > 0.0)) fail(GLX_EINVAL, "theta must be > 0");
-  if (p->leaf_size > 1) fail(GLX_EINVAL, "leaf_size must be 1");
+  if (p->leaf_size > kMaxLeafSize) fail(GLX_EINVAL, "leaf_size must be at most 32");
+  if (p->leaf_size > 1 && p->walk != GLX_WALK_GROUP)
+    fail(GLX_EINVAL, "leaf_size > 1 needs the group walk");
   if (p->multipole != 0 && p->multipole != 2) fail(GLX_EINVAL, "multipole must be 0 or 2");
   if (p->precision > GLX_PREC_KAHAN) fail(GLX_EINVAL, "unknown precision mode");
   if (p->morton_bits != 0 && p->morton_bits != 30 && p->morton_bits != 63)
@@ -72,6 +75,7 @@ inline void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
   d.eps2 = (float)(p.softening * p.softening);
   d.theta = (float)p.theta;
   d.precision = p.precision;
+  d.leaf_size = p.leaf_size ? p.leaf_size : 1u;
 }
 
 inline hipStream_t make_stream() {

@@ -83,6 +83,7 @@ struct SegDesc {
   uint32_t n;
   uint32_t sort_tile0, sort_tiles;  // element tiles (kSortTile) of this segment
   uint32_t precision;
+  uint32_t leaf_size;  // group walk: nodes of at most this many bodies are summed body by body
   float g, eps2, theta, half_dt;
   double dt;
 };

@@ -881,6 +881,33 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
 // the per-target walk.
 constexpr int kStackCap = 1536;
 constexpr int kListCap = 128;
+constexpr int kRangeCap = 128;
+
+// Point-mass term of one source on the lane's target (self and coincident sources add 0).
+template <bool KAHAN>
+__device__ __forceinline__ void point_term(Acc3& a, const float4& me, const float4& p, float eps2) {
+  const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
+  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+  const float ir = rsqrtf(r2);
+  const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
+  acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
+}
+
+// Bodies of the buckets in `rl[0, nr)` (first leaf, count) on the lane's target. The
+// range is wave-uniform, so the body records come through scalar loads.
+template <bool KAHAN>
+__device__ __forceinline__ uint32_t bucket_terms(Acc3& a, const float4& me, const SegDesc& s,
+                                                 const uint2* rl, uint32_t nr, float eps2) {
+  uint32_t total = 0;
+  for (uint32_t r = 0; r < nr; ++r) {
+    const uint2 rg = rl[r];
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(rg.x);
+    const uint32_t c = __builtin_amdgcn_readfirstlane(rg.y);
+    for (uint32_t e = 0; e < c; ++e) point_term<KAHAN>(a, me, s.leaf[lo + e], eps2);
+    total += c;
+  }
+  return total;
+}
 
 template <bool KAHAN, bool COUNT, bool QUAD>
 __global__ void __launch_bounds__(kGroup)
@@ -891,6 +918,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
   __shared__ float4 pl[kListCap];
   __shared__ float4 qa[QN], qb[QN];
   __shared__ float2 qc[QN];
+  __shared__ uint2 rl[kRangeCap];
   const uint32_t lane = threadIdx.x;
   const unsigned long long lt = (1ull << lane) - 1ull;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -904,9 +932,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const float eps2 = s.eps2;
     const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
     const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
+    const uint32_t bucket = s.leaf_size;
     Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     unsigned long long n_int = 0, n_vis = 0, n_quad = 0;
-    uint32_t sp = 1, np = 0, nq = 0;
+    uint32_t sp = 1, np = 0, nq = 0, nr = 0;
     bool overflow = false;
     if (lane == 0) stk[0] = (n == 1) ? 0u : n;
     __syncthreads();
@@ -925,7 +954,8 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       // Classify without divergent branches: every lane loads a valid leaf and node
       // record (index 0 stands in for the other kind) and the action is selected, so
       // the ballots below always see the full wave.
-      // act: 0 nothing, 1 point/monopole, 2 quadrupole node, 3 open
+      // act: 0 nothing, 1 point/monopole, 2 quadrupole node, 3 open, 4 bucket (a node
+      // of at most leaf_size bodies that is not accepted: its bodies are summed directly)
       const bool live = u != kEnd;
       const bool is_leaf = live && u < n;
       const bool is_node = live && u >= n;
@@ -943,9 +973,12 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
                          (Cc.z < gz0) | (B.z > gz1);
       const bool node_live = is_node & (A.w > 0.0f);
       const bool accept = node_live & apart & (dmin2 >= B.w);
-      const bool open = node_live & !accept;
+      const uint32_t nlo = nd.d.z, ncount = nd.d.w - nd.d.z + 1u;
+      const bool near = node_live & !accept;
+      const bool bkt = near & (ncount <= bucket);
+      const bool open = near & !bkt;
       const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
-      const int act = point ? 1 : (accept ? 2 : (open ? 3 : 0));
+      const int act = point ? 1 : (accept ? 2 : (open ? 3 : (bkt ? 4 : 0)));
       const float4 P = is_leaf ? LP : A;
       float4 Q0 = make_float4(0.f, 0.f, 0.f, 0.f);
       float2 Q1 = make_float2(0.f, 0.f);
@@ -956,7 +989,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       }
       const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
       const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
-                               m3 = __ballot(act == 3);
+                               m3 = __ballot(act == 3), m4 = __ballot(act == 4);
       const uint32_t c3 = (uint32_t)__popcll(m3);
       if (sp + 2u * c3 > (uint32_t)kStackCap) {
         overflow = true;
@@ -974,21 +1007,21 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         stk[q] = cr;
         stk[q + 1] = cl;  // left child popped first
       }
+      if (act == 4) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
       np += (uint32_t)__popcll(m1);
       nq += (uint32_t)__popcll(m2);
+      nr += (uint32_t)__popcll(m4);
       sp += 2u * c3;
       __syncthreads();
       if (np >= 64u) {
-        for (uint32_t e = 0; e < np; ++e) {
-          const float4 p = pl[e];
-          const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
-          const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-          const float ir = rsqrtf(r2);
-          const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
-          acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
-        }
+        for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
         if (COUNT) n_int += np;
         np = 0;
+      }
+      if (nr >= 64u) {
+        const uint32_t c = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
+        if (COUNT) n_int += c;
+        nr = 0;
       }
       if (QUAD && nq >= 64u) {
         for (uint32_t e = 0; e < nq; ++e) {
@@ -1017,14 +1050,9 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
       if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
     } else {
-      for (uint32_t e = 0; e < np; ++e) {
-        const float4 p = pl[e];
-        const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
-        const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-        const float ir = rsqrtf(r2);
-        const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
-        acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
-      }
+      for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
+      const uint32_t cb = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
+      if (COUNT) n_int += cb;
       if (QUAD) {
         for (uint32_t e = 0; e < nq; ++e) {
           const float4 A = qa[e], B = qb[e];

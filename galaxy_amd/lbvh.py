@@ -150,8 +150,9 @@ def device_info():
 
 
 def _params(g, softening, theta, precision=PREC_F32, count=False, morton_bits=30,
-            walk=WALK_REFERENCE, multipole=0) -> Params:
-    return Params(float(g), float(softening), float(theta), 1, int(multipole), int(precision),
+            walk=WALK_REFERENCE, multipole=0, leaf_size=1) -> Params:
+    return Params(float(g), float(softening), float(theta), int(leaf_size), int(multipole),
+                  int(precision),
                   FLAG_COUNT if count else 0, int(morton_bits), int(walk))
 
 
@@ -174,10 +175,11 @@ class Lbvh:
 
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
                  count: bool = False, morton_bits: int = 30, walk: int = WALK_REFERENCE,
-                 multipole: int = 0):
+                 multipole: int = 0, leaf_size: int = 1):
         self.g, self.softening, self.theta = float(g), float(softening), float(theta)
         h = C.c_void_p()
-        p = _params(g, softening, theta, precision, count, morton_bits, walk, multipole)
+        p = _params(g, softening, theta, precision, count, morton_bits, walk, multipole,
+                    leaf_size)
         _check(lib().glx_solver_create(C.byref(p), C.byref(h)))
         self._h = h
 
@@ -322,9 +324,11 @@ class GpuResidentLeapfrog:
     snapshot. Positions FP64 on the device, velocities and forces FP32."""
 
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
-                 morton_bits: int = 30, walk: int = WALK_REFERENCE, multipole: int = 0):
+                 morton_bits: int = 30, walk: int = WALK_REFERENCE, multipole: int = 0,
+                 leaf_size: int = 1):
         h = C.c_void_p()
-        p = _params(g, softening, theta, precision, False, morton_bits, walk, multipole)
+        p = _params(g, softening, theta, precision, False, morton_bits, walk, multipole,
+                    leaf_size)
         _check(lib().glx_sys_create(C.byref(p), C.byref(h)))
         self._h = h
         self._mass = np.zeros(0)

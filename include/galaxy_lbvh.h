@@ -48,7 +48,9 @@ typedef struct glx_params {
   double g;             /* gravitational constant */
   double softening;     /* Plummer softening length eps (> = 0) */
   double theta;         /* opening angle (> 0) */
-  uint32_t leaf_size;   /* bodies per leaf; must be 1 (the reference's LBVH leaf) */
+  uint32_t leaf_size;   /* bodies per leaf, 0 or 1 = the reference's LBVH leaf; up to 32
+                           with GLX_WALK_GROUP: a node of at most leaf_size bodies that
+                           fails the opening test is summed body by body */
   uint32_t multipole;   /* 0 = monopole (the reference's expansion), 2 = + quadrupole */
   uint32_t precision;   /* GLX_PREC_* */
   uint32_t flags;       /* GLX_FLAG_* */

@@ -69,7 +69,8 @@ def test_invalid_params_rejected_before_device():
         L.Params(1.0, 0.05, 0.0, 1, 0, 0, 0),        # theta = 0
         L.Params(1.0, -1.0, 0.5, 1, 0, 0, 0),        # softening < 0
         L.Params(float("nan"), 0.05, 0.5, 1, 0, 0, 0),
-        L.Params(1.0, 0.05, 0.5, 4, 0, 0, 0),        # leaf_size
+        L.Params(1.0, 0.05, 0.5, 4, 0, 0, 0),        # leaf_size > 1 needs the group walk
+        L.Params(1.0, 0.05, 0.5, 33, 0, 0, 0, 63, 1),  # leaf_size above 32
         L.Params(1.0, 0.05, 0.5, 1, 1, 0, 0),        # multipole
         L.Params(1.0, 0.05, 0.5, 1, 0, 7, 0),        # precision
     ):

@@ -141,6 +141,40 @@ def _errs(a, ref):
     return I.rel_err_quantiles(a, ref)
 
 
+@pytest.mark.parametrize("leaf", [2, 8, 32])
+def test_leaf_size_theta_zero_and_small_systems_are_direct_sum(leaf):
+    """Buckets are summed body by body, so theta -> 0 stays the direct sum, and a system
+    of at most leaf_size bodies is the direct sum at any theta."""
+    for kw in (FAST, GROUP_MONO):
+        s = glx.Lbvh(G, EPS, 1e-6, leaf_size=leaf, **kw)
+        for seed in range(4):
+            pos, _, mass = I.cluster(seed, 300)
+            assert I.rms_rel_err(s.accelerations(pos, mass), O.direct_sum(pos, mass, G, EPS)) < 2e-5
+        small = glx.Lbvh(G, EPS, 0.9, leaf_size=leaf, **kw)
+        pos, _, mass = I.cluster(7, leaf)
+        assert I.rms_rel_err(small.accelerations(pos, mass), O.direct_sum(pos, mass, G, EPS)) < 2e-6
+
+
+def test_leaf_size_error_and_work():
+    """Plummer N=16384, theta=0.5: larger buckets never raise the error (a bucket is
+    exact where leaf size 1 might accept a descendant) and cut node tests."""
+    pos, _, mass = I.plummer_numpy(16384, seed=9)
+    eps = 0.01
+    tg = np.arange(16384, dtype=np.uint32)
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, eps, tg)
+    prev = None
+    for leaf in (1, 4, 16):
+        s = glx.Lbvh(1.0, eps, 0.5, count=True, leaf_size=leaf, **FAST)
+        e = _errs(s.accelerations(pos, mass), d64)
+        st = s.stats()
+        say(f"leaf {leaf}: err {e} stats {st}")
+        assert st["fallbacks"] == 0
+        if prev is not None:
+            assert e[0] <= prev[0][0] * 1.01 and e[1] <= prev[0][1] * 1.01
+            assert st["node_tests"] < prev[1]["node_tests"]
+        prev = (e, st)
+
+
 def test_group_walk_error_not_worse_than_reference_walk():
     """Plummer N=16384, eps=0.01: per-particle relative error vs FP64 direct sum.
     Group monopole <= reference per-target walk (same theta); quadrupole at least 3x
