@@ -471,6 +471,7 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       lf.z = (float)(s.x[3 * i + 2] - o2);
       lf.w = (float)s.m[i];
       s.leaf[k] = lf;
+      s.gend[k] = s.n == 1 ? 0u : kEnd;  // group starts are marked by the tree build
       if (s.n == 1) s.parent_leaf[0] = kNoParent;
     }
   }
@@ -529,6 +530,14 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       s.lo[i] = (uint32_t)lo;
       s.hi[i] = (uint32_t)hi;
       if (i == 0) s.parent_int[0] = kNoParent;
+      // walk groups: the subtrees of at most kGroup bodies whose parent holds more;
+      // a node above the limit marks those of its two children
+      if (hi - lo + 1 > kGroup) {
+        if (gamma - lo + 1 <= kGroup) s.gend[lo] = (uint32_t)gamma;
+        if (hi - gamma <= kGroup) s.gend[gamma + 1] = (uint32_t)hi;
+      } else if (i == 0) {
+        s.gend[0] = (uint32_t)(n - 1);
+      }
     }
   }
 }
@@ -926,167 +935,179 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const SegDesc& s = segs[tl.seg];
     if (tl.start == 0 && lane < 6) bbox_neutral(s.bbox, lane);
     const uint32_t n = s.n;
-    const bool valid = lane < tl.count;
-    const uint32_t k = tl.start + (valid ? lane : 0u);
-    const float4 me = s.leaf[k];
     const float eps2 = s.eps2;
-    const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
-    const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
     const uint32_t bucket = s.leaf_size;
-    Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    unsigned long long n_int = 0, n_vis = 0, n_quad = 0;
-    uint32_t sp = 1, np = 0, nq = 0, nr = 0;
-    bool overflow = false;
-    if (lane == 0) stk[0] = (n == 1) ? 0u : n;
-    __syncthreads();
-    // every node is pushed at most once, so a round count above 2n only happens on a
-    // corrupted tree; the bound keeps such input from spinning forever
-    for (uint32_t round = 0; sp > 0; ++round) {
-      if (round > 2 * n + 2) {
-        overflow = true;
-        break;
-      }
-      const uint32_t m = sp < 64u ? sp : 64u;
-      const uint32_t base = sp - m;
-      const uint32_t u = lane < m ? stk[base + lane] : kEnd;
-      sp = base;
-      if (COUNT && lane == 0) n_vis += m;
-      // Classify without divergent branches: every lane loads a valid leaf and node
-      // record (index 0 stands in for the other kind) and the action is selected, so
-      // the ballots below always see the full wave.
-      // act: 0 nothing, 1 point/monopole, 2 quadrupole node, 3 open, 4 bucket (a node
-      // of at most leaf_size bodies that is not accepted: its bodies are summed directly)
-      const bool live = u != kEnd;
-      const bool is_leaf = live && u < n;
-      const bool is_node = live && u >= n;
-      const uint32_t li = is_leaf ? u : 0u;
-      const uint32_t ni = is_node ? u - n : 0u;
-      const float4 LP = s.leaf[li];
-      const Node& nd = s.node[n > 1 ? ni : 0u];
-      const float4 A = nd.a, B = nd.b, Cc = nd.c;
-      const uint32_t crn = nd.d.y;
-      const float ex = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
-      const float ey = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
-      const float ez = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
-      const float dmin2 = ex * ex + ey * ey + ez * ez;
-      const bool apart = (Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) | (B.y > gy1) |
-                         (Cc.z < gz0) | (B.z > gz1);
-      const bool node_live = is_node & (A.w > 0.0f);
-      const bool accept = node_live & apart & (dmin2 >= B.w);
-      const uint32_t nlo = nd.d.z, ncount = nd.d.w - nd.d.z + 1u;
-      const bool near = node_live & !accept;
-      const bool bkt = near & (ncount <= bucket);
-      const bool open = near & !bkt;
-      const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
-      const int act = point ? 1 : (accept ? 2 : (open ? 3 : (bkt ? 4 : 0)));
-      const float4 P = is_leaf ? LP : A;
-      float4 Q0 = make_float4(0.f, 0.f, 0.f, 0.f);
-      float2 Q1 = make_float2(0.f, 0.f);
-      if (QUAD) {
-        const float4 q1 = s.quad[2 * ni + 1];
-        Q0 = s.quad[2 * ni];
-        Q1 = make_float2(q1.x, q1.y);
-      }
-      const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
-      const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
-                               m3 = __ballot(act == 3), m4 = __ballot(act == 4);
-      const uint32_t c3 = (uint32_t)__popcll(m3);
-      if (sp + 2u * c3 > (uint32_t)kStackCap) {
-        overflow = true;
-        break;
-      }
-      if (act == 1) pl[np + __popcll(m1 & lt)] = P;
-      if (QUAD && act == 2) {
-        const uint32_t q = nq + __popcll(m2 & lt);
-        qa[q] = P;
-        qb[q] = Q0;
-        qc[q] = Q1;
-      }
-      if (act == 3) {
-        const uint32_t q = sp + 2u * (uint32_t)__popcll(m3 & lt);
-        stk[q] = cr;
-        stk[q + 1] = cl;  // left child popped first
-      }
-      if (act == 4) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
-      np += (uint32_t)__popcll(m1);
-      nq += (uint32_t)__popcll(m2);
-      nr += (uint32_t)__popcll(m4);
-      sp += 2u * c3;
+    // the walk groups (subtrees of at most kGroup bodies, marked by the tree build)
+    // that start in this chunk of sorted leaves, one after the other
+    const uint32_t ge = lane < tl.count ? s.gend[tl.start + lane] : kEnd;
+    unsigned long long starts = __ballot(ge != kEnd);
+    unsigned long long t_int = 0, t_vis = 0, t_quad = 0;
+    while (starts != 0ull) {
+      const uint32_t l0 = (uint32_t)__builtin_ctzll(starts);
+      starts &= starts - 1ull;
+      const uint32_t g0 = tl.start + l0;
+      const uint32_t gl = (uint32_t)__shfl((int)ge, (int)l0);
+      const uint32_t gcount = min(min(gl - g0 + 1u, (uint32_t)kGroup), n - g0);
+      const bool valid = lane < gcount;
+      const uint32_t k = g0 + (valid ? lane : 0u);
+      const float4 me = s.leaf[k];
+      const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
+      const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
+      Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      unsigned long long n_int = 0, n_vis = 0, n_quad = 0;
+      uint32_t sp = 1, np = 0, nq = 0, nr = 0;
+      bool overflow = false;
+      if (lane == 0) stk[0] = (n == 1) ? 0u : n;
       __syncthreads();
-      if (np >= 64u) {
+      // every node is pushed at most once, so a round count above 2n only happens on a
+      // corrupted tree; the bound keeps such input from spinning forever
+      for (uint32_t round = 0; sp > 0; ++round) {
+        if (round > 2 * n + 2) {
+          overflow = true;
+          break;
+        }
+        const uint32_t m = sp < 64u ? sp : 64u;
+        const uint32_t base = sp - m;
+        const uint32_t u = lane < m ? stk[base + lane] : kEnd;
+        sp = base;
+        if (COUNT && lane == 0) n_vis += m;
+        // Classify without divergent branches: every lane loads a valid leaf and node
+        // record (index 0 stands in for the other kind) and the action is selected, so
+        // the ballots below always see the full wave.
+        // act: 0 nothing, 1 point/monopole, 2 quadrupole node, 3 open, 4 bucket (a node
+        // of at most leaf_size bodies that is not accepted: its bodies are summed directly)
+        const bool live = u != kEnd;
+        const bool is_leaf = live && u < n;
+        const bool is_node = live && u >= n;
+        const uint32_t li = is_leaf ? u : 0u;
+        const uint32_t ni = is_node ? u - n : 0u;
+        const float4 LP = s.leaf[li];
+        const Node& nd = s.node[n > 1 ? ni : 0u];
+        const float4 A = nd.a, B = nd.b, Cc = nd.c;
+        const uint32_t crn = nd.d.y;
+        const float ex = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
+        const float ey = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
+        const float ez = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
+        const float dmin2 = ex * ex + ey * ey + ez * ez;
+        const bool apart = (Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) | (B.y > gy1) |
+                           (Cc.z < gz0) | (B.z > gz1);
+        const bool node_live = is_node & (A.w > 0.0f);
+        const bool accept = node_live & apart & (dmin2 >= B.w);
+        const uint32_t nlo = nd.d.z, ncount = nd.d.w - nd.d.z + 1u;
+        const bool near = node_live & !accept;
+        const bool bkt = near & (ncount <= bucket);
+        const bool open = near & !bkt;
+        const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
+        const int act = point ? 1 : (accept ? 2 : (open ? 3 : (bkt ? 4 : 0)));
+        const float4 P = is_leaf ? LP : A;
+        float4 Q0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 Q1 = make_float2(0.f, 0.f);
+        if (QUAD) {
+          const float4 q1 = s.quad[2 * ni + 1];
+          Q0 = s.quad[2 * ni];
+          Q1 = make_float2(q1.x, q1.y);
+        }
+        const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
+        const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
+                                 m3 = __ballot(act == 3), m4 = __ballot(act == 4);
+        const uint32_t c3 = (uint32_t)__popcll(m3);
+        if (sp + 2u * c3 > (uint32_t)kStackCap) {
+          overflow = true;
+          break;
+        }
+        if (act == 1) pl[np + __popcll(m1 & lt)] = P;
+        if (QUAD && act == 2) {
+          const uint32_t q = nq + __popcll(m2 & lt);
+          qa[q] = P;
+          qb[q] = Q0;
+          qc[q] = Q1;
+        }
+        if (act == 3) {
+          const uint32_t q = sp + 2u * (uint32_t)__popcll(m3 & lt);
+          stk[q] = cr;
+          stk[q + 1] = cl;  // left child popped first
+        }
+        if (act == 4) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
+        np += (uint32_t)__popcll(m1);
+        nq += (uint32_t)__popcll(m2);
+        nr += (uint32_t)__popcll(m4);
+        sp += 2u * c3;
+        __syncthreads();
+        if (np >= 64u) {
+          for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
+          if (COUNT) n_int += np;
+          np = 0;
+        }
+        if (nr >= 64u) {
+          const uint32_t c = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
+          if (COUNT) n_int += c;
+          nr = 0;
+        }
+        if (QUAD && nq >= 64u) {
+          for (uint32_t e = 0; e < nq; ++e) {
+            const float4 A = qa[e], B = qb[e];
+            const float2 Cq = qc[e];
+            const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
+            const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
+                        ir7 = ir5 * ir2;
+            const float qx = B.x * dx + B.w * dy + Cq.x * dz;
+            const float qy = B.w * dx + B.y * dy + Cq.y * dz;
+            const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
+            const float dqd = dx * qx + dy * qy + dz * qz;
+            const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
+            acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
+          }
+          if (COUNT) n_quad += nq;
+          nq = 0;
+        }
+        __syncthreads();
+      }
+      if (overflow) {
+        a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        n_int = 0;
+        n_quad = 0;
+        if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
+        if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
+      } else {
         for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
-        if (COUNT) n_int += np;
-        np = 0;
-      }
-      if (nr >= 64u) {
-        const uint32_t c = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
-        if (COUNT) n_int += c;
-        nr = 0;
-      }
-      if (QUAD && nq >= 64u) {
-        for (uint32_t e = 0; e < nq; ++e) {
-          const float4 A = qa[e], B = qb[e];
-          const float2 Cq = qc[e];
-          const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
-          const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-          const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
-                      ir7 = ir5 * ir2;
-          const float qx = B.x * dx + B.w * dy + Cq.x * dz;
-          const float qy = B.w * dx + B.y * dy + Cq.y * dz;
-          const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
-          const float dqd = dx * qx + dy * qy + dz * qz;
-          const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
-          acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
+        const uint32_t cb = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
+        if (COUNT) n_int += cb;
+        if (QUAD) {
+          for (uint32_t e = 0; e < nq; ++e) {
+            const float4 A = qa[e], B = qb[e];
+            const float2 Cq = qc[e];
+            const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
+            const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
+                        ir7 = ir5 * ir2;
+            const float qx = B.x * dx + B.w * dy + Cq.x * dz;
+            const float qy = B.w * dx + B.y * dy + Cq.y * dz;
+            const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
+            const float dqd = dx * qx + dy * qy + dz * qz;
+            const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
+            acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
+          }
         }
-        if (COUNT) n_quad += nq;
-        nq = 0;
+        if (COUNT) {
+          n_int += np;
+          n_quad += nq;
+        }
+      }
+      if (valid) store_acc<KAHAN>(s, k, a, mode);
+      if (COUNT) {
+        // n_int is per target (uniform over valid lanes unless the group fell back)
+        t_int += wave_sum_u64(valid ? n_int : 0ull);
+        t_quad += wave_sum_u64(valid ? n_quad : 0ull);
+        t_vis += wave_sum_u64(n_vis);
       }
       __syncthreads();
     }
-    if (overflow) {
-      a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      n_int = 0;
-      n_quad = 0;
-      if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
-      if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
-    } else {
-      for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
-      const uint32_t cb = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
-      if (COUNT) n_int += cb;
-      if (QUAD) {
-        for (uint32_t e = 0; e < nq; ++e) {
-          const float4 A = qa[e], B = qb[e];
-          const float2 Cq = qc[e];
-          const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
-          const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-          const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
-                      ir7 = ir5 * ir2;
-          const float qx = B.x * dx + B.w * dy + Cq.x * dz;
-          const float qy = B.w * dx + B.y * dy + Cq.y * dz;
-          const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
-          const float dqd = dx * qx + dy * qy + dz * qz;
-          const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
-          acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
-        }
-      }
-      if (COUNT) {
-        n_int += np;
-        n_quad += nq;
-      }
+    if (COUNT && lane == 0 && (t_int | t_quad | t_vis)) {
+      atomicAdd(&s.counters[kCntInteractions], t_int);
+      atomicAdd(&s.counters[kCntQuad], t_quad);
+      atomicAdd(&s.counters[kCntVisits], t_vis);
     }
-    if (valid) store_acc<KAHAN>(s, k, a, mode);
-    if (COUNT) {
-      // n_int is per target (uniform over valid lanes unless the group fell back)
-      const unsigned long long vi = wave_sum_u64(valid ? n_int : 0ull);
-      const unsigned long long vq = wave_sum_u64(valid ? n_quad : 0ull);
-      const unsigned long long vv = wave_sum_u64(n_vis);
-      if (lane == 0) {
-        atomicAdd(&s.counters[kCntInteractions], vi);
-        atomicAdd(&s.counters[kCntQuad], vq);
-        atomicAdd(&s.counters[kCntVisits], vv);
-      }
-    }
-    __syncthreads();
   }
 }
 

@@ -55,7 +55,7 @@ struct Carve {
 
 struct Layout {
   size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, quad, left, right, pleaf, pint, rs, lo,
-      hi, flag, roots, bbox, bounds, counters, total;
+      hi, flag, roots, gend, bbox, bounds, counters, total;
 };
 
 Layout layout_for(uint32_t n, bool with_acc_out) {
@@ -83,6 +83,7 @@ Layout layout_for(uint32_t n, bool with_acc_out) {
   L.hi = c.take<uint32_t>(N);
   L.flag = c.take<uint32_t>(N);
   L.roots = c.take<uint32_t>(N);
+  L.gend = c.take<uint32_t>(N);
   L.bbox = c.take<unsigned long long>(6);
   L.bounds = c.take<double>(8);
   L.counters = c.take<unsigned long long>(kCounters);
@@ -143,6 +144,7 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   d.hi = reinterpret_cast<uint32_t*>(b + L.hi);
   d.flag = reinterpret_cast<uint32_t*>(b + L.flag);
   d.roots = reinterpret_cast<uint32_t*>(b + L.roots);
+  d.gend = reinterpret_cast<uint32_t*>(b + L.gend);
   d.bbox = reinterpret_cast<unsigned long long*>(b + L.bbox);
   d.bounds = reinterpret_cast<double*>(b + L.bounds);
   d.counters = reinterpret_cast<unsigned long long*>(b + L.counters);
