@@ -188,7 +188,7 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
                  "point_terms_per_particle": cnt["point_terms"] / n_tot,
                  "quad_terms_per_particle": cnt["quad_terms"] / n_tot,
                  "node_tests_per_particle": cnt["node_tests"] / n_tot,
-                 "fallbacks": cnt["fallbacks"]},
+                 "fallbacks": cnt["fallbacks"], "max_stack": cnt["max_stack"]},
         "error": quantiles(np.concatenate(errs)),
         "admit_s": t_admit,
         "particles": n_tot,
@@ -246,7 +246,7 @@ def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets
                  "point_terms_per_particle": c0["point_terms"] / n,
                  "quad_terms_per_particle": c0["quad_terms"] / n,
                  "node_tests_per_particle": c0["node_tests"] / n,
-                 "fallbacks": c0["fallbacks"]},
+                 "fallbacks": c0["fallbacks"], "max_stack": c0["max_stack"]},
         "error": quantiles(rel_err(acc0[d64_targets], d64)),
     }
 
@@ -415,12 +415,26 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="job arrivals per second")
     ap.add_argument("--profile", action="store_true",
                     help="pool steps only (for profilers): no plain, collision or CPU legs")
+    ap.add_argument("--profile-collision", action="store_true",
+                    help="collision fast build only (for profilers)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 3 times (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU; runs on several GPUs are "
                                    "not part of it (independent systems would shard across "
                                    "GPUs with no communication)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return
     if args.profile:
         args.no_cpu = args.no_plain = args.no_collision = True
 
@@ -430,6 +444,13 @@ def main():
 
     fast_mode = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
     ref_mode = dict(morton_bits=30, walk=glx.WALK_REFERENCE, multipole=0)
+    if args.profile_collision:
+        n = args.collision_n
+        pos, vel, mass, _ = ic.collision(n, args.seed)
+        tg = np.arange(min(n, 256), dtype=np.uint32)
+        f = single(glx, pos, vel, mass, 0.05, 0.6, 5e-3, args.collision_steps, 1, fast_mode, tg)
+        print(json.dumps({"collision_2p20_fast": strip(f)}), flush=True)
+        return
     eps, theta, dt = 0.01, 0.5, 1e-3
 
     sizes = ensemble_sizes(args.seed, args.systems)

@@ -117,7 +117,7 @@ inline void sync(hipStream_t st) { check(hipStreamSynchronize(st), "hipStreamSyn
 inline void zero_walk_counters(const SegDesc& d, hipStream_t st) {
   check(hipMemsetAsync(d.counters + kCntInteractions, 0, 2 * sizeof(unsigned long long), st),
         "memset counters");
-  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 2 * sizeof(unsigned long long), st),
+  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 3 * sizeof(unsigned long long), st),
         "memset counters");
 }
 
@@ -127,6 +127,7 @@ inline glx_stats stats_from(const unsigned long long* c) {
   s.quad_terms = c[kCntQuad];
   s.node_tests = c[kCntVisits];
   s.fallbacks = c[kCntOverflow];
+  s.max_stack = c[kCntMaxStack];
   return s;
 }
 

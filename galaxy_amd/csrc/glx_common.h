@@ -52,6 +52,7 @@ enum : int {
   kCntRoots = 2,         // local subtree roots handed to the top bottom-up pass
   kCntOverflow = 3,      // group walks that fell back to the per-target walk
   kCntQuad = 4,          // quadrupole-node terms, per target
+  kCntMaxStack = 5,      // deepest group-walk stack, entries
   kCounters = 8
 };
 

@@ -336,6 +336,7 @@ k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg, const uint32_t* __
     const uint32_t nt = s.sort_tiles, per = (nt + kScanChunks - 1) / kScanChunks;
     const uint32_t c0 = s.sort_tile0 + min(nt, g * per), c1 = s.sort_tile0 + min(nt, (g + 1) * per);
     uint32_t sum = 0;
+#pragma unroll 8
     for (uint32_t t = c0; t < c1; ++t) sum += tile_hist[(size_t)t * kRadixBins + d];
     part[g][d] = sum;
     __syncthreads();
@@ -355,6 +356,7 @@ k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg, const uint32_t* __
     uint32_t run = base[d];
     for (uint32_t w = 0; w < (d >> 6); ++w) run += wsum[w];
     for (uint32_t q = 0; q < g; ++q) run += part[q][d];
+#pragma unroll 8
     for (uint32_t t = c0; t < c1; ++t) {
       const uint32_t h = tile_hist[(size_t)t * kRadixBins + d];
       tile_off[(size_t)t * kRadixBins + d] = run;
@@ -652,8 +654,12 @@ __device__ __forceinline__ AggV global_child(const SegDesc& s, uint32_t c) {
   return r;
 }
 
+// One thread per leaf of the tile: every climb starts at once, so the critical path
+// is the depth of the tile's subtrees.
+constexpr int kAggThreads = kAggTile;
+
 template <bool QUAD>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kAggThreads)
 k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
                  uint32_t ntiles) {
   constexpr int NQ = QUAD ? 6 : 1;
@@ -667,9 +673,9 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
     if (n < 2) continue;
     const uint32_t t0 = tl.start, t1 = tl.start + tl.count;
     const float inv_theta = 1.0f / s.theta;
-    for (uint32_t q = threadIdx.x; q < kAggTile; q += kBlock) s_flag[q] = 0u;
+    for (uint32_t q = threadIdx.x; q < kAggTile; q += kAggThreads) s_flag[q] = 0u;
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
+    for (uint32_t e = threadIdx.x; e < tl.count; e += kAggThreads) {
       const uint32_t k = t0 + e;
       uint32_t u = k;
       uint32_t p = s.parent_leaf[k];
@@ -717,32 +723,51 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
   }
 }
 
+constexpr int kTopThreads = 1024;
+
+// Each lane climbs from one handed-over root at a time and takes the next root from a
+// workgroup counter as soon as its climb ends (first arrival at a node), so the block
+// stays busy and the critical path is the height of the cross-tile part of the tree.
 template <bool QUAD>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kTopThreads)
 k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
+  __shared__ uint32_t next_root;
   for (uint32_t sidx = blockIdx.x; sidx < nseg; sidx += gridDim.x) {
     const SegDesc& s = segs[sidx];
     const uint32_t n = s.n;
+    if (threadIdx.x == 0) next_root = 0u;
+    __syncthreads();
     if (n >= 2) {
       const float inv_theta = 1.0f / s.theta;
       const unsigned long long cnt64 = s.counters[kCntRoots];
       const uint32_t cnt = cnt64 < n ? (uint32_t)cnt64 : n;
-      for (uint32_t r = threadIdx.x; r < cnt; r += kBlock) {
-        const uint32_t u = s.roots[r];
-        uint32_t p = u < n ? s.parent_leaf[u] : s.parent_int[u - n];
-        for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {
-          const uint32_t j = p - n;
-          if (j >= n - 1) break;
-          const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (old == 0u) break;
-          __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, s.left[j]),
-                                        global_child<QUAD>(s, s.right[j]));
-          store_node<QUAD>(s, j, a, inv_theta);
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-          p = s.parent_int[j];
+      uint32_t p = kNoParent;
+      // a lane takes at most cnt roots and climbs at most n-1 nodes in all
+      for (uint32_t guard = 0; guard < cnt + n; ++guard) {
+        if (p == kNoParent) {
+          const uint32_t r = atomicAdd(&next_root, 1u);
+          if (r >= cnt) break;
+          const uint32_t u = s.roots[r];
+          p = u < n ? s.parent_leaf[u] : s.parent_int[u - n];
+          continue;
         }
+        const uint32_t j = p - n;
+        if (j >= n - 1) {
+          p = kNoParent;
+          continue;
+        }
+        const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) {
+          p = kNoParent;
+          continue;
+        }
+        __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, s.left[j]),
+                                      global_child<QUAD>(s, s.right[j]));
+        store_node<QUAD>(s, j, a, inv_theta);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        p = s.parent_int[j];
       }
     }
     __syncthreads();
@@ -889,8 +914,8 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
 // nodes push both children. If the stack would overflow, the group falls back to
 // the per-target walk.
 constexpr int kStackCap = 1536;
-constexpr int kListCap = 128;
-constexpr int kRangeCap = 128;
+constexpr int kListCap = 64;   // lists are drained every round, which adds at most 64
+constexpr int kRangeCap = 64;
 
 // Point-mass term of one source on the lane's target (self and coincident sources add 0).
 template <bool KAHAN>
@@ -900,6 +925,22 @@ __device__ __forceinline__ void point_term(Acc3& a, const float4& me, const floa
   const float ir = rsqrtf(r2);
   const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
   acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
+}
+
+// Monopole + quadrupole term of one node (com/mass P, traceless Q = {xx,yy,zz,xy}, {xz,yz})
+// with the softened radius: a = G[m d / r^3 - Q d / r^5 + 2.5 (d.Q.d) d / r^7], d = com - x.
+template <bool KAHAN>
+__device__ __forceinline__ void quad_term(Acc3& a, const float4& me, const float4& P,
+                                          const float4& B, const float2& Cq, float eps2) {
+  const float dx = P.x - me.x, dy = P.y - me.y, dz = P.z - me.z;
+  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+  const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2;
+  const float qx = B.x * dx + B.w * dy + Cq.x * dz;
+  const float qy = B.w * dx + B.y * dy + Cq.y * dz;
+  const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
+  const float dqd = dx * qx + dy * qy + dz * qz;
+  const float c0 = P.w * ir3 + 2.5f * dqd * ir7;
+  acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
 }
 
 // Bodies of the buckets in `rl[0, nr)` (first leaf, count) on the lane's target. The
@@ -942,6 +983,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const uint32_t ge = lane < tl.count ? s.gend[tl.start + lane] : kEnd;
     unsigned long long starts = __ballot(ge != kEnd);
     unsigned long long t_int = 0, t_vis = 0, t_quad = 0;
+    uint32_t t_peak = 0;
     while (starts != 0ull) {
       const uint32_t l0 = (uint32_t)__builtin_ctzll(starts);
       starts &= starts - 1ull;
@@ -955,7 +997,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
       Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       unsigned long long n_int = 0, n_vis = 0, n_quad = 0;
-      uint32_t sp = 1, np = 0, nq = 0, nr = 0;
+      uint32_t sp = 1, np = 0, nq = 0, nr = 0, peak = 1;
       bool overflow = false;
       if (lane == 0) stk[0] = (n == 1) ? 0u : n;
       __syncthreads();
@@ -1032,32 +1074,20 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         nq += (uint32_t)__popcll(m2);
         nr += (uint32_t)__popcll(m4);
         sp += 2u * c3;
+        peak = sp > peak ? sp : peak;
         __syncthreads();
-        if (np >= 64u) {
+        if (np != 0u) {
           for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
           if (COUNT) n_int += np;
           np = 0;
         }
-        if (nr >= 64u) {
+        if (nr != 0u) {
           const uint32_t c = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
           if (COUNT) n_int += c;
           nr = 0;
         }
-        if (QUAD && nq >= 64u) {
-          for (uint32_t e = 0; e < nq; ++e) {
-            const float4 A = qa[e], B = qb[e];
-            const float2 Cq = qc[e];
-            const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
-            const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-            const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
-                        ir7 = ir5 * ir2;
-            const float qx = B.x * dx + B.w * dy + Cq.x * dz;
-            const float qy = B.w * dx + B.y * dy + Cq.y * dz;
-            const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
-            const float dqd = dx * qx + dy * qy + dz * qz;
-            const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
-            acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
-          }
+        if (QUAD && nq != 0u) {
+          for (uint32_t e = 0; e < nq; ++e) quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
           if (COUNT) n_quad += nq;
           nq = 0;
         }
@@ -1069,37 +1099,14 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         n_quad = 0;
         if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
         if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
-      } else {
-        for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
-        const uint32_t cb = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
-        if (COUNT) n_int += cb;
-        if (QUAD) {
-          for (uint32_t e = 0; e < nq; ++e) {
-            const float4 A = qa[e], B = qb[e];
-            const float2 Cq = qc[e];
-            const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
-            const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-            const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2,
-                        ir7 = ir5 * ir2;
-            const float qx = B.x * dx + B.w * dy + Cq.x * dz;
-            const float qy = B.w * dx + B.y * dy + Cq.y * dz;
-            const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
-            const float dqd = dx * qx + dy * qy + dz * qz;
-            const float c0 = A.w * ir3 + 2.5f * dqd * ir7;
-            acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
-          }
-        }
-        if (COUNT) {
-          n_int += np;
-          n_quad += nq;
-        }
-      }
+      }  // otherwise the lists were drained in the last round
       if (valid) store_acc<KAHAN>(s, k, a, mode);
       if (COUNT) {
         // n_int is per target (uniform over valid lanes unless the group fell back)
         t_int += wave_sum_u64(valid ? n_int : 0ull);
         t_quad += wave_sum_u64(valid ? n_quad : 0ull);
         t_vis += wave_sum_u64(n_vis);
+        t_peak = peak > t_peak ? peak : t_peak;
       }
       __syncthreads();
     }
@@ -1107,6 +1114,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       atomicAdd(&s.counters[kCntInteractions], t_int);
       atomicAdd(&s.counters[kCntQuad], t_quad);
       atomicAdd(&s.counters[kCntVisits], t_vis);
+      atomicMax(&s.counters[kCntMaxStack], (unsigned long long)t_peak);
     }
   }
 }
@@ -1357,16 +1365,13 @@ hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, co
 
 hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
                             const Tile* at, uint32_t nat) {
+  const dim3 tg(grid_for(nat)), tb(kAggThreads), sg(grid_for(nseg)), sb(kTopThreads);
   if (md.quad) {
-    hipLaunchKernelGGL(k_aggregate_tile<true>, dim3(grid_for(nat)), dim3(kBlock), 0, st, segs, at,
-                       nat);
-    hipLaunchKernelGGL(k_aggregate_top<true>, dim3(grid_for(nseg)), dim3(kBlock), 0, st, segs,
-                       nseg);
+    hipLaunchKernelGGL(k_aggregate_tile<true>, tg, tb, 0, st, segs, at, nat);
+    hipLaunchKernelGGL(k_aggregate_top<true>, sg, sb, 0, st, segs, nseg);
   } else {
-    hipLaunchKernelGGL(k_aggregate_tile<false>, dim3(grid_for(nat)), dim3(kBlock), 0, st, segs, at,
-                       nat);
-    hipLaunchKernelGGL(k_aggregate_top<false>, dim3(grid_for(nseg)), dim3(kBlock), 0, st, segs,
-                       nseg);
+    hipLaunchKernelGGL(k_aggregate_tile<false>, tg, tb, 0, st, segs, at, nat);
+    hipLaunchKernelGGL(k_aggregate_top<false>, sg, sb, 0, st, segs, nseg);
   }
   return hipGetLastError();
 }

@@ -203,6 +203,7 @@ extern "C" int glx_pool_count(glx_pool* p, glx_stats* out) {
       out->quad_terms += s.quad_terms;
       out->node_tests += s.node_tests;
       out->fallbacks += s.fallbacks;
+      out->max_stack = s.max_stack > out->max_stack ? s.max_stack : out->max_stack;
     }
   });
 }

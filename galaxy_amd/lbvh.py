@@ -63,6 +63,7 @@ class Stats(C.Structure):
         ("quad_terms", C.c_uint64),
         ("node_tests", C.c_uint64),
         ("fallbacks", C.c_uint64),
+        ("max_stack", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

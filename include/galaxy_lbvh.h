@@ -76,6 +76,7 @@ typedef struct glx_stats {
   uint64_t quad_terms;  /* quadrupole-node terms */
   uint64_t node_tests;  /* nodes tested (per-target walk: per target; group walk: per group) */
   uint64_t fallbacks;   /* group walks that fell back to the per-target walk */
+  uint64_t max_stack;   /* deepest group-walk node stack, entries */
 } glx_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -91,8 +92,9 @@ int glx_device_info(char *name, size_t name_len, int *compute_units, uint64_t *h
 typedef struct glx_solver glx_solver;
 
 /* Lbvh::new / GpuLbvhFused::new (solvers/src/lbvh.rs:653; gpu/src/gpu_lbvh_fused.rs:50).
- * GLX_EINVAL for theta <= 0, softening < 0, non-finite g/eps/theta, leaf_size > 1,
- * unknown multipole/precision/morton_bits/walk; GLX_EDEVICE without a GPU. */
+ * GLX_EINVAL for theta <= 0, softening < 0, non-finite g/eps/theta, leaf_size > 32 (or
+ * > 1 without GLX_WALK_GROUP), unknown multipole/precision/morton_bits/walk;
+ * GLX_EDEVICE without a GPU. */
 int glx_solver_create(const glx_params *params, glx_solver **out);
 int glx_solver_destroy(glx_solver *solver);
 
