@@ -913,34 +913,57 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
 // evaluate against their own targets (LDS broadcast reads, no divergence). Opened
 // nodes push both children. If the stack would overflow, the group falls back to
 // the per-target walk.
-constexpr int kStackCap = 1536;
+constexpr int kStackCap = 1024;
 constexpr int kListCap = 64;   // lists are drained every round, which adds at most 64
 constexpr int kRangeCap = 64;
 
-// Point-mass term of one source on the lane's target (self and coincident sources add 0).
+// Floor on the softened r^2 of a point term: a source at the target's position (dx = 0
+// exactly, the target itself included) then adds 0 * finite rather than 0 * inf when the
+// softening is 0. It is far below any separation f32 positions resolve at unit scale.
+constexpr float kR2Floor = 1e-20f;
+
+// Point-mass term of one source on the lane's target: a += m d / (|d|^2 + eps^2)^1.5.
+// The raw v_rsq (1 ulp) serves as rsqrt: r^2 is never denormal here.
 template <bool KAHAN>
 __device__ __forceinline__ void point_term(Acc3& a, const float4& me, const float4& p, float eps2) {
   const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
-  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-  const float ir = rsqrtf(r2);
-  const float w = r2 > 0.0f ? p.w * ir * ir * ir : 0.0f;
-  acc_add<KAHAN>(a, dx * w, dy * w, dz * w);
+  const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))), kR2Floor);
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float w = p.w * (ir * ir) * ir;
+  if (KAHAN) {
+    acc_add<true>(a, dx * w, dy * w, dz * w);
+  } else {
+    a.x = fmaf(dx, w, a.x);
+    a.y = fmaf(dy, w, a.y);
+    a.z = fmaf(dz, w, a.z);
+  }
 }
 
 // Monopole + quadrupole term of one node (com/mass P, traceless Q = {xx,yy,zz,xy}, {xz,yz})
-// with the softened radius: a = G[m d / r^3 - Q d / r^5 + 2.5 (d.Q.d) d / r^7], d = com - x.
+// with the softened radius r^2 = |d|^2 + eps^2, d = com - x:
+//   a = m d / r^3 - Q d / r^5 + 2.5 (d.Q.d) d / r^7
+//     = r^-5 [(m r^2 + 2.5 (d.Q.d) / r^2) d - Q d]
+// An accepted node is never at the target (its box is disjoint from the group's), so
+// r^2 > 0.
 template <bool KAHAN>
 __device__ __forceinline__ void quad_term(Acc3& a, const float4& me, const float4& P,
                                           const float4& B, const float2& Cq, float eps2) {
   const float dx = P.x - me.x, dy = P.y - me.y, dz = P.z - me.z;
-  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-  const float ir = rsqrtf(r2), ir2 = ir * ir, ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2;
-  const float qx = B.x * dx + B.w * dy + Cq.x * dz;
-  const float qy = B.w * dx + B.y * dy + Cq.y * dz;
-  const float qz = Cq.x * dx + Cq.y * dy + B.z * dz;
-  const float dqd = dx * qx + dy * qy + dz * qz;
-  const float c0 = P.w * ir3 + 2.5f * dqd * ir7;
-  acc_add<KAHAN>(a, c0 * dx - ir5 * qx, c0 * dy - ir5 * qy, c0 * dz - ir5 * qz);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+  const float ir = __builtin_amdgcn_rsqf(r2), ir2 = ir * ir, ir5 = (ir2 * ir2) * ir;
+  const float qx = fmaf(B.x, dx, fmaf(B.w, dy, Cq.x * dz));
+  const float qy = fmaf(B.w, dx, fmaf(B.y, dy, Cq.y * dz));
+  const float qz = fmaf(Cq.x, dx, fmaf(Cq.y, dy, B.z * dz));
+  const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+  const float c = fmaf(P.w, r2, 2.5f * dqd * ir2);
+  const float tx = fmaf(c, dx, -qx), ty = fmaf(c, dy, -qy), tz = fmaf(c, dz, -qz);
+  if (KAHAN) {
+    acc_add<true>(a, ir5 * tx, ir5 * ty, ir5 * tz);
+  } else {
+    a.x = fmaf(ir5, tx, a.x);
+    a.y = fmaf(ir5, ty, a.y);
+    a.z = fmaf(ir5, tz, a.z);
+  }
 }
 
 // Bodies of the buckets in `rl[0, nr)` (first leaf, count) on the lane's target. The
@@ -1077,6 +1100,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         peak = sp > peak ? sp : peak;
         __syncthreads();
         if (np != 0u) {
+#pragma unroll 2
           for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
           if (COUNT) n_int += np;
           np = 0;
@@ -1087,6 +1111,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           nr = 0;
         }
         if (QUAD && nq != 0u) {
+#pragma unroll 2
           for (uint32_t e = 0; e < nq; ++e) quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
           if (COUNT) n_quad += nq;
           nq = 0;
