@@ -413,6 +413,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=0.0,
                     help="also run the serving stream for this long (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="job arrivals per second")
+    ap.add_argument("--leaf-size", type=int, default=0,
+                    help="walk leaf size of the ensemble (0: autotune on 64 of its systems)")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--precision-only", action="store_true", help="only the precision stress")
     ap.add_argument("--profile", action="store_true",
                     help="pool steps only (for profilers): no plain, collision or CPU legs")
     ap.add_argument("--profile-collision", action="store_true",
@@ -444,6 +448,17 @@ def main():
 
     fast_mode = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
     ref_mode = dict(morton_bits=30, walk=glx.WALK_REFERENCE, multipole=0)
+    if args.serve_only or args.precision_only:
+        out = {}
+        if args.serve_only:
+            out["serving"] = serve(glx, ic, Pool, args.serve_seconds or 60.0, args.serve_rate,
+                                   args.seed)
+        if args.precision_only:
+            out["precision_stress"] = precision(glx, ic, args.precision_n or (1 << 23), args.seed,
+                                                args.precision_steps, diag_every=100)
+        out["device"] = glx.device_info()
+        print(json.dumps(out), flush=True)
+        return
     if args.profile_collision:
         n = args.collision_n
         pos, vel, mass, _ = ic.collision(n, args.seed)
@@ -458,6 +473,18 @@ def main():
     systems = [ic.plummer(int(n), args.seed + 1 + k) for k, n in enumerate(sizes)]
     log(f"ensemble: {len(systems)} systems, {int(sizes.sum())} particles, "
         f"ICs {time.perf_counter() - t0:.1f}s")
+    tuning = None
+    if args.leaf_size:
+        leaf = args.leaf_size
+    else:
+        from galaxy_amd.tune import tune_leaf_size
+
+        pick = np.random.default_rng(args.seed + 7).choice(len(systems), 64, replace=False)
+        tuning = tune_leaf_size([systems[k] for k in sorted(pick)], softening=eps, theta=theta,
+                                dt=dt)
+        leaf = tuning["leaf_size"]
+        log(f"tuned leaf size {leaf}: {tuning['table']}")
+    fast_mode["leaf_size"] = leaf
     b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, fast_mode, 8)
     log(f"batched: {b['value']:.4g} pf/s, {b['ms_per_step']:.2f} ms/step, err {b['error']}")
     out = {
@@ -478,7 +505,7 @@ def main():
                    "systems": len(systems), "particles": int(sizes.sum()),
                    "n_range": [1024, 65536], "theta": theta, "eps": eps, "dt": dt,
                    "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
-                   "leaf_size": 1},
+                   "leaf_size": leaf},
         "p50_step_ms": b["device_ms"]["p50"],
         "p99_step_ms": b["device_ms"]["p99"],
         "force_error_vs_fp64_direct": b["error"],
@@ -495,6 +522,7 @@ def main():
             "walk_ms": b["walk"]["ms"],
         },
         "batched": strip(b),
+        "leaf_size_tuning": tuning,
     }
     if not args.no_plain:
         p = plain(glx, systems, eps, theta, dt, args.plain_steps, fast_mode)
