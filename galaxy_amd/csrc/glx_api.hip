@@ -1,6 +1,10 @@
 // C ABI of the LBVH gravity step (include/galaxy_lbvh.h). Every entry point converts
 // runtime failures into a GLX_* status plus a thread-local message.
+#include <stdio.h>
+
+#include "galaxy_sim.h"
 #include "glx_abi_util.h"
+#include "glx_snapshot.h"
 
 using namespace glx;
 using namespace glx::abi;
@@ -420,25 +424,30 @@ extern "C" int glx_sys_upload(glx_sys* s, size_t n, const double* pos, const dou
   });
 }
 
+// k KDK steps of dt through the captured step graph (re-captured when dt changes).
+static void sys_step(glx_sys* s, double dt, uint64_t k) {
+  if (!isfinite(dt)) fail(GLX_EINVAL, "dt must be finite");
+  if (s->n == 0) {
+    s->time += (double)k * dt;
+    return;
+  }
+  if (k == 0) return;
+  if (!(dt == s->cur_dt)) {
+    s->batch->set_dt(dt);
+    s->cur_dt = dt;
+  }
+  if (!s->graph.exec) s->graph.capture(*s->batch);
+  for (uint64_t q = 0; q < k; ++q) {
+    s->graph.launch(s->st);
+    s->time += dt;
+  }
+  sync(s->st);
+}
+
 extern "C" int glx_sys_step(glx_sys* s, double dt, uint64_t k) {
   return guard([&] {
     if (!s) fail(GLX_EINVAL, "sys is NULL");
-    if (!isfinite(dt)) fail(GLX_EINVAL, "dt must be finite");
-    if (s->n == 0) {
-      s->time += (double)k * dt;
-      return;
-    }
-    if (k == 0) return;
-    if (!(dt == s->cur_dt)) {
-      s->batch->set_dt(dt);
-      s->cur_dt = dt;
-    }
-    if (!s->graph.exec) s->graph.capture(*s->batch);
-    for (uint64_t q = 0; q < k; ++q) {
-      s->graph.launch(s->st);
-      s->time += dt;
-    }
-    sync(s->st);
+    sys_step(s, dt, k);
   });
 }
 
@@ -539,6 +548,78 @@ extern "C" int glx_sys_diag(glx_sys* s, int with_potential, glx_diag* out) {
             "potential");
       d2h(&out->potential, part.p + (nb - 1), sizeof(double), s->st);
       sync(s->st);
+    }
+  });
+}
+
+extern "C" int glx_sys_set_time(glx_sys* s, double time) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "sys is NULL");
+    if (!isfinite(time)) fail(GLX_EINVAL, "time must be finite");
+    s->time = time;
+  });
+}
+
+static const char kCodeVersion[] = "galaxy_amd 0.1.0";
+
+// sim::run (sim/src/lib.rs:110-174): the state as uploaded is snapshot `first_step`, then
+// steps advance in runs that end on the cadence (one sync per run), a snapshot follows
+// every multiple of snapshot_every, and the final step is always written.
+extern "C" int glx_sys_run(glx_sys* s, const glx_run_config* cfg, const char* dir,
+                           const glx_snap_columns* cols, glx_run_summary* out) {
+  return guard([&] {
+    if (!s || !cfg || !dir) fail(GLX_EINVAL, "NULL argument");
+    if (cfg->snapshot_every == 0) fail(GLX_EINVAL, "snapshot_every must be >= 1");
+    if (!isfinite(cfg->dt) || !(cfg->dt > 0.0))
+      fail(GLX_EINVAL, "dt must be a positive finite number");
+    const size_t n = s->n;
+    std::vector<double> pos(3 * n), vel(3 * n), mass(n);
+    std::vector<float> v32(3 * n);
+    if (n) {
+      d2h(mass.data(), s->seg.desc.m, n * sizeof(double), s->st);
+      sync(s->st);
+    }
+    uint64_t emitted = 0;
+    auto emit = [&](uint64_t step) {
+      if (n) {
+        d2h(pos.data(), s->seg.desc.x, 3 * n * sizeof(double), s->st);
+        d2h(v32.data(), s->seg.desc.v, 3 * n * sizeof(float), s->st);
+        sync(s->st);
+        for (size_t q = 0; q < 3 * n; ++q) vel[q] = (double)v32[q];
+      }
+      glx_snap_header h{};
+      h.time = s->time;
+      h.step = cfg->first_step + step;
+      h.scale_factor = 1.0;
+      h.softening = cfg->softening;
+      h.rng_seed = cfg->rng_seed;
+      h.config_hash = cfg->config_hash;
+      h.units = cfg->units ? cfg->units : "";
+      h.code_version = kCodeVersion;
+      char name[48];
+      snprintf(name, sizeof(name), "snapshot_%08llu.snap", (unsigned long long)h.step);
+      const std::string path = std::string(dir) + "/" + name;
+      glx::snap::write(path.c_str(), h, n, pos.data(), vel.data(), mass.data(), cols);
+      ++emitted;
+    };
+    emit(0);
+    uint64_t step = 0, last = 0;
+    const uint64_t every = cfg->snapshot_every;
+    while (step < cfg->n_steps) {
+      const uint64_t cadence = (step / every + 1) * every;
+      const uint64_t next = cadence < cfg->n_steps ? cadence : cfg->n_steps;
+      sys_step(s, cfg->dt, next - step);
+      step = next;
+      if (step % every == 0) {
+        emit(step);
+        last = step;
+      }
+    }
+    if (last != cfg->n_steps) emit(cfg->n_steps);
+    if (out) {
+      out->steps = cfg->n_steps;
+      out->final_time = s->time;
+      out->snapshots_emitted = emitted;
     }
   });
 }

@@ -40,9 +40,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_binding_covers_header():
-    from galaxy_amd import ic, pool
+    from galaxy_amd import ic, pool, sim
     pool._bind()
     ic._bind()
+    sim._bind()
     for n in declared_functions():
         assert getattr(L.lib(), n).argtypes is not None or n == "glx_last_error", n
 
