@@ -783,6 +783,18 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
 //    (core/src/integrator.rs:70-73).
 // ----------------------------------------------------------------------------------
 
+// MI355X deals the blocks of a launch round-robin over its 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2. Walk groups that are neighbours in Morton order share most
+// of the nodes they read, so the walks number their blocks XCD-major: logical block
+// (b % 8) * (G / 8) + b / 8 puts consecutive tiles on one XCD and in one L2. Launches use
+// a grid that is a multiple of 8 (xcd_grid); any other grid keeps the hardware order.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  return (g % kXcds) ? b : (b % kXcds) * (g / kXcds) + b / kXcds;
+}
+
 struct Acc3 {
   float x, y, z, cx, cy, cz;
 };
@@ -886,7 +898,7 @@ template <bool KAHAN, bool COUNT>
 __global__ void __launch_bounds__(kBlock)
 k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
             int mode) {
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     if (tl.start == 0 && threadIdx.x < 6) bbox_neutral(s.bbox, threadIdx.x);
@@ -994,7 +1006,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
   __shared__ uint2 rl[kRangeCap];
   const uint32_t lane = threadIdx.x;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     if (tl.start == 0 && lane < 6) bbox_neutral(s.bbox, lane);
@@ -1310,6 +1322,13 @@ static inline uint32_t grid_for(uint32_t ntiles, uint32_t cap = 65535u) {
   return ntiles == 0 ? 1u : (ntiles < cap ? ntiles : cap);
 }
 
+// Grid for the XCD-major walks: a multiple of 8 blocks (idle blocks skip the tile loop).
+static inline uint32_t xcd_grid(uint32_t ntiles) {
+  if (ntiles < kXcds) return ntiles == 0 ? 1u : ntiles;
+  const uint32_t g = (ntiles + kXcds - 1) / kXcds * kXcds;
+  return g < 65528u ? g : 65528u;
+}
+
 hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg) {
   if (nseg == 0) return hipSuccess;
   hipLaunchKernelGGL(k_seg_reset, dim3((nseg * 6 + 255) / 256), dim3(256), 0, st, segs, nseg);
@@ -1405,10 +1424,10 @@ template <bool KAHAN, bool COUNT>
 static void walk_group_q(hipStream_t st, bool quad, const SegDesc* segs, const Tile* gt,
                          uint32_t ngt, int mode) {
   if (quad)
-    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true>), dim3(grid_for(ngt)), dim3(kGroup), 0, st,
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true>), dim3(xcd_grid(ngt)), dim3(kGroup), 0, st,
                        segs, gt, ngt, mode);
   else
-    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false>), dim3(grid_for(ngt)), dim3(kGroup), 0,
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false>), dim3(xcd_grid(ngt)), dim3(kGroup), 0,
                        st, segs, gt, ngt, mode);
 }
 
@@ -1424,7 +1443,7 @@ hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, cons
     }
     return hipGetLastError();
   }
-  const dim3 g(grid_for(nwt)), b(kBlock);
+  const dim3 g(xcd_grid(nwt)), b(kBlock);
   if (md.kahan) {
     if (count)
       hipLaunchKernelGGL((k_walk_lane<true, true>), g, b, 0, st, segs, wt, nwt, mode);
