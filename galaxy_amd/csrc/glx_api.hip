@@ -41,9 +41,13 @@ struct glx_solver {
   glx_stats stats{};
   double* pot_partial = nullptr;
   size_t pot_cap = 0;
+  uint8_t* mask = nullptr;  // walk mask of the active calls (original order)
+  size_t mask_cap = 0;
+  uint32_t cache_n = 0;     // bodies of the tree kept by the last build (0: none)
   ~glx_solver() {
     delete batch;
     (void)hipFree(pot_partial);
+    (void)hipFree(mask);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -86,6 +90,9 @@ static void solver_stage_inputs(glx_solver* s, size_t n, const double* pos, cons
   h2d(s->seg.desc.m, mass, n * sizeof(double), s->st);
 }
 
+static void solver_eval(glx_solver* s, size_t n, const double* pos, const double* mass,
+                        const size_t* active, size_t n_active, double* acc, bool cached);
+
 extern "C" int glx_solver_accelerations(glx_solver* s, size_t n, const double* pos,
                                         const double* mass, double* acc) {
   return guard([&] {
@@ -98,27 +105,124 @@ extern "C" int glx_solver_accelerations(glx_solver* s, size_t n, const double* p
       acc[0] = acc[1] = acc[2] = 0.0;
       return;
     }
+    solver_eval(s, n, pos, mass, nullptr, 0, acc, false);
+  });
+}
+
+// Uploads the walk mask of `active` (original indices), or clears it for a full walk.
+static void solver_set_mask(glx_solver* s, size_t n, const size_t* active, size_t n_active) {
+  if (!active) {
+    s->batch->set_active(nullptr);
+    return;
+  }
+  std::vector<uint8_t> m(n, 0);
+  for (size_t q = 0; q < n_active; ++q) {
+    if (active[q] >= n) fail(GLX_EINVAL, "active index out of range");
+    m[active[q]] = 1;
+  }
+  if (s->mask_cap < n) {
+    (void)hipFree(s->mask);
+    s->mask = nullptr;
+    s->mask_cap = 0;
+    check(hipMalloc(&s->mask, n), "hipMalloc");
+    s->mask_cap = n;
+  }
+  h2d(s->mask, m.data(), n, s->st);
+  s->batch->set_active(s->mask);
+}
+
+// One force evaluation of n >= 2 bodies: a fresh build (kept as the cached tree) or, with
+// `cached`, the kept tree with the leaves moved to the new positions; walks every group
+// (active == null) or the groups holding an active body.
+static void solver_eval(glx_solver* s, size_t n, const double* pos, const double* mass,
+                        const size_t* active, size_t n_active, double* acc, bool cached) {
+  if (cached && s->cache_n != n) fail(GLX_EINVAL, "no cached tree of n bodies");
+  if (cached) {
+    h2d(s->seg.desc.x, pos, 3 * n * sizeof(double), s->st);
+    h2d(s->seg.desc.m, mass, n * sizeof(double), s->st);
+  } else {
+    s->cache_n = 0;
     solver_stage_inputs(s, n, pos, mass);
-    const bool count = (s->p.flags & GLX_FLAG_COUNT) != 0;
-    if (count) zero_walk_counters(s->seg.desc, s->st);
-    Batch& b = *s->batch;
+  }
+  const bool count = (s->p.flags & GLX_FLAG_COUNT) != 0;
+  if (count) zero_walk_counters(s->seg.desc, s->st);
+  Batch& b = *s->batch;
+  solver_set_mask(s, n, active, n_active);
+  if (cached) {
+    b.refresh_leaves();
+    b.walk(0, count);
+  } else {
     b.reset_bbox();
     b.bbox();
     b.build_and_walk(0, count);
-    d2h(acc, s->seg.desc.acc_out, 3 * n * sizeof(double), s->st);
-    unsigned long long c[kCounters] = {};
-    if (count) d2h(c, s->seg.desc.counters, sizeof(c), s->st);
-    sync(s->st);
-    s->stats = stats_from(c);
-  });
+  }
+  d2h(acc, s->seg.desc.acc_out, 3 * n * sizeof(double), s->st);
+  unsigned long long c[kCounters] = {};
+  if (count) d2h(c, s->seg.desc.counters, sizeof(c), s->st);
+  sync(s->st);
+  if (active) b.set_active(nullptr);
+  s->stats = stats_from(c);
+  s->cache_n = (uint32_t)n;
 }
 
 extern "C" int glx_solver_accelerations_active(glx_solver* s, size_t n, const double* pos,
                                                const double* mass, const size_t* active,
                                                size_t n_active, double* acc) {
-  (void)active;
-  (void)n_active;
-  return glx_solver_accelerations(s, n, pos, mass, acc);
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "solver is NULL");
+    validate_n(n);
+    if (n == 0) return;
+    if (!pos || !mass || !acc || (n_active && !active)) fail(GLX_EINVAL, "NULL array");
+    s->stats = glx_stats{};
+    if (n == 1) {
+      acc[0] = acc[1] = acc[2] = 0.0;
+      return;
+    }
+    solver_eval(s, n, pos, mass, active ? active : nullptr, n_active, acc, false);
+  });
+}
+
+extern "C" int glx_solver_rebuild_cache(glx_solver* s, size_t n, const double* pos,
+                                        const double* mass) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "solver is NULL");
+    validate_n(n);
+    s->cache_n = 0;
+    if (n < 2) {
+      s->cache_n = (uint32_t)n;
+      return;
+    }
+    if (!pos || !mass) fail(GLX_EINVAL, "NULL array");
+    solver_stage_inputs(s, n, pos, mass);
+    Batch& b = *s->batch;
+    b.reset_bbox();
+    b.bbox();
+    b.encode();
+    b.sort();
+    b.gather();
+    b.karras();
+    b.aggregate();
+    sync(s->st);
+    s->cache_n = (uint32_t)n;
+  });
+}
+
+extern "C" int glx_solver_gravity_active_cached(glx_solver* s, size_t n, const double* pos,
+                                                const double* mass, const size_t* active,
+                                                size_t n_active, double* acc) {
+  return guard([&] {
+    if (!s) fail(GLX_EINVAL, "solver is NULL");
+    validate_n(n);
+    if (n != s->cache_n) fail(GLX_EINVAL, "no cached tree of n bodies");
+    if (n == 0) return;
+    if (!pos || !mass || !acc || (n_active && !active)) fail(GLX_EINVAL, "NULL array");
+    s->stats = glx_stats{};
+    if (n == 1) {
+      acc[0] = acc[1] = acc[2] = 0.0;
+      return;
+    }
+    solver_eval(s, n, pos, mass, active, n_active, acc, true);
+  });
 }
 
 extern "C" int glx_solver_stats(glx_solver* s, glx_stats* out) {

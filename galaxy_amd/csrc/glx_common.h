@@ -79,6 +79,7 @@ struct SegDesc {
   uint32_t* flag;        // n-1 arrival counters of the top bottom-up pass (kept at 0)
   uint32_t* roots;       // n local subtree roots handed from the tile pass to the top pass
   uint32_t* gend;        // n walk groups: last leaf of the group starting at k, else kEnd
+  const uint8_t* active; // optional walk mask over the original indices (null: walk all)
   unsigned long long* bbox;      // 6 order-preserving encodings: min xyz, max xyz
   double* bounds;                // 8: bmin xyz, size, scale, origin xyz
   unsigned long long* counters;  // kCounters: see kCnt*

@@ -17,6 +17,8 @@ hipError_t launch_iota(hipStream_t st, const SegDesc* segs, const Tile* et, uint
 hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
                        const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off);
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
+                                 uint32_t net);
 hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net);
 hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,

@@ -458,6 +458,9 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
 //    (solvers/src/lbvh.rs:453-456; gpu/src/fused_core.rs:32-45)
 // ----------------------------------------------------------------------------------
 
+// MARKS: a fresh build (also clears the walk-group marks); without: the positions of a
+// cached tree are refreshed in its existing leaf order.
+template <bool MARKS>
 __global__ void __launch_bounds__(kBlock)
 k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -473,8 +476,10 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       lf.z = (float)(s.x[3 * i + 2] - o2);
       lf.w = (float)s.m[i];
       s.leaf[k] = lf;
-      s.gend[k] = s.n == 1 ? 0u : kEnd;  // group starts are marked by the tree build
-      if (s.n == 1) s.parent_leaf[0] = kNoParent;
+      if (MARKS) {
+        s.gend[k] = s.n == 1 ? 0u : kEnd;  // group starts are marked by the tree build
+        if (s.n == 1) s.parent_leaf[0] = kNoParent;
+      }
     }
   }
 }
@@ -904,6 +909,7 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
     if (tl.start == 0 && threadIdx.x < 6) bbox_neutral(s.bbox, threadIdx.x);
     if (threadIdx.x >= tl.count) continue;
     const uint32_t k = tl.start + threadIdx.x;
+    if (s.active && !s.active[min(s.vals[0][k], s.n - 1)]) continue;
     Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     unsigned long long n_int = 0, n_vis = 0;
     lane_walk<KAHAN, COUNT>(s, k, s.leaf[k], a, n_int, n_vis);
@@ -1027,6 +1033,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       const uint32_t gcount = min(min(gl - g0 + 1u, (uint32_t)kGroup), n - g0);
       const bool valid = lane < gcount;
       const uint32_t k = g0 + (valid ? lane : 0u);
+      if (s.active) {  // masked walk: a group without an active body is skipped
+        const bool on = valid && s.active[min(s.vals[0][k], n - 1)] != 0;
+        if (__ballot(on) == 0ull) continue;
+      }
       const float4 me = s.leaf[k];
       const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
       const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
@@ -1394,7 +1404,13 @@ hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint
 }
 
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather<true>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  return hipGetLastError();
+}
+
+hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
+                                 uint32_t net) {
+  hipLaunchKernelGGL(k_gather<false>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
   return hipGetLastError();
 }
 

@@ -57,6 +57,8 @@ class Batch {
   void set_segments(const std::vector<SegDesc>& descs);
   // Patch per-segment step parameters (dt) on the device without rebuilding tables.
   void set_dt(double dt);
+  // Patch the walk mask of every segment (null: walk all).
+  void set_active(const uint8_t* mask);
   void set_mode(const Mode& m) { mode_ = m; }
   const Mode& mode() const { return mode_; }
   uint32_t nseg() const { return nseg_; }
@@ -72,6 +74,7 @@ class Batch {
   void iota();
   void sort();
   void gather();
+  void refresh_leaves();  // new positions into the cached leaf order (no rebuild)
   void karras();
   void aggregate();
   void walk(int mode, bool count);

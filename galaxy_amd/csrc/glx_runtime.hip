@@ -248,6 +248,15 @@ void Batch::set_dt(double dt) {
   check(hipStreamSynchronize(stream_), "set_dt");
 }
 
+void Batch::set_active(const uint8_t* mask) {
+  for (auto& d : host_) d.active = mask;
+  if (nseg_)
+    check(hipMemcpyAsync(segs_, host_.data(), nseg_ * sizeof(SegDesc), hipMemcpyHostToDevice,
+                         stream_),
+          "upload segments");
+  check(hipStreamSynchronize(stream_), "set_active");
+}
+
 void Batch::reset_bbox() { check(launch_seg_reset(stream_, segs_, nseg_), "seg_reset"); }
 void Batch::bbox() { check(launch_bbox(stream_, segs_, etiles_, n_et_), "bbox"); }
 void Batch::kick_drift_bbox() {
@@ -262,6 +271,9 @@ void Batch::sort() {
   check(launch_sort(stream_, mode_, segs_, nseg_, etiles_, n_et_, tile_hist_, tile_off_), "sort");
 }
 void Batch::gather() { check(launch_gather(stream_, segs_, etiles_, n_et_), "gather"); }
+void Batch::refresh_leaves() {
+  check(launch_refresh_leaves(stream_, segs_, etiles_, n_et_), "refresh leaves");
+}
 void Batch::karras() { check(launch_karras(stream_, mode_, segs_, etiles_, n_et_), "karras"); }
 void Batch::aggregate() {
   check(launch_aggregate(stream_, mode_, segs_, nseg_, atiles_, n_at_), "aggregate");

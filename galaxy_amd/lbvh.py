@@ -94,6 +94,8 @@ def lib():
         "glx_solver_destroy": (I, [P]),
         "glx_solver_accelerations": (I, [P, S, P, P, P]),
         "glx_solver_accelerations_active": (I, [P, S, P, P, P, S, P]),
+        "glx_solver_rebuild_cache": (I, [P, S, P, P]),
+        "glx_solver_gravity_active_cached": (I, [P, S, P, P, P, S, P]),
         "glx_solver_potential_energy": (I, [P, S, P, P, C.POINTER(D)]),
         "glx_solver_stats": (I, [P, C.POINTER(Stats)]),
         "glx_direct_sum_f64": (I, [S, P, P, D, D, P, S, P]),
@@ -207,6 +209,34 @@ class Lbvh:
             raise ValueError("acc length must match particle count")
         _check(lib().glx_solver_accelerations(self._h, n, _p(pos), _p(mass), _p(acc)))
         return acc
+
+    def _active_call(self, fn, pos, mass, active):
+        pos = _f64(pos, (-1, 3))
+        mass = _f64(mass)
+        n = mass.size
+        if pos.shape[0] != n:
+            raise ValueError("pos and mass lengths differ")
+        act = None if active is None else np.ascontiguousarray(active, dtype=np.uintp)
+        acc = np.zeros((n, 3))
+        _check(fn(self._h, n, _p(pos), _p(mass), None if act is None else _p(act),
+                  0 if act is None else act.size, _p(acc)))
+        return acc
+
+    def accelerations_active(self, pos, mass, active) -> np.ndarray:
+        """ForceSolver::accelerations_active (core/src/traits.rs:43-54): rows in `active`
+        equal the full pass; the other rows are unspecified."""
+        return self._active_call(lib().glx_solver_accelerations_active, pos, mass, active)
+
+    def rebuild_gravity_cache(self, pos, mass):
+        """ForceSolver::rebuild_gravity_cache (core/src/traits.rs:73-78)."""
+        pos = _f64(pos, (-1, 3))
+        mass = _f64(mass)
+        _check(lib().glx_solver_rebuild_cache(self._h, mass.size, _p(pos), _p(mass)))
+
+    def gravity_active_cached(self, pos, mass, active=None) -> np.ndarray:
+        """ForceSolver::gravity_active_cached (core/src/traits.rs:80-91): the kept tree
+        walked at the current positions for the `active` rows."""
+        return self._active_call(lib().glx_solver_gravity_active_cached, pos, mass, active)
 
     def potential_energy(self, pos, mass) -> float:
         pos = _f64(pos, (-1, 3))

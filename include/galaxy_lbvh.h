@@ -105,11 +105,26 @@ int glx_solver_destroy(glx_solver *solver);
 int glx_solver_accelerations(glx_solver *solver, size_t n, const double *pos,
                              const double *mass, double *acc);
 
-/* ForceSolver::accelerations_active default (core/src/traits.rs:52-54): computes
- * the full pass; entries of acc outside `active` are filled as well. */
+/* ForceSolver::accelerations_active (core/src/traits.rs:43-54): a fresh tree of all n
+ * bodies, walked only for the groups that hold an active body, so each active entry
+ * equals the full pass bit for bit; entries outside `active` are unspecified (the
+ * trait's contract). active == NULL walks every body. Index >= n: GLX_EINVAL. */
 int glx_solver_accelerations_active(glx_solver *solver, size_t n, const double *pos,
                                     const double *mass, const size_t *active,
                                     size_t n_active, double *acc);
+
+/* ForceSolver::rebuild_gravity_cache (core/src/traits.rs:73-78): builds the tree of these
+ * positions and keeps it (every accelerations call also leaves its tree cached). */
+int glx_solver_rebuild_cache(glx_solver *solver, size_t n, const double *pos,
+                             const double *mass);
+
+/* ForceSolver::gravity_active_cached (core/src/traits.rs:80-91): walks the kept tree for
+ * the active bodies at the current positions — bodies of opened nodes at `pos`, accepted
+ * nodes with the multipoles and bounds of the kept build. Entries outside `active` are
+ * unspecified; GLX_EINVAL when no tree of n bodies is kept. */
+int glx_solver_gravity_active_cached(glx_solver *solver, size_t n, const double *pos,
+                                     const double *mass, const size_t *active,
+                                     size_t n_active, double *acc);
 
 /* ForceSolver::potential_energy (core/src/traits.rs:95; solvers/src/lbvh.rs:680-682):
  * exact O(n^2) FP64 softened potential, computed on the GPU. */
