@@ -8,7 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/galaxy_lbvh.h"
+#include "galaxy_lbvh.h"
 #include "glx_runtime.h"
 
 namespace glx {

@@ -17,7 +17,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/galaxy_ic.h"
+#include "galaxy_ic.h"
 
 namespace {
 

@@ -789,15 +789,21 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
 // ----------------------------------------------------------------------------------
 
 // MI355X deals the blocks of a launch round-robin over its 8 XCDs (block b runs on XCD
-// b % 8), each with its own L2. Walk groups that are neighbours in Morton order share most
-// of the nodes they read, so the walks number their blocks XCD-major: logical block
-// (b % 8) * (G / 8) + b / 8 puts consecutive tiles on one XCD and in one L2. Launches use
-// a grid that is a multiple of 8 (xcd_grid); any other grid keeps the hardware order.
+// b % 8), each with its own L2. Walk groups that are neighbours in Morton order read
+// nearly the same nodes, so the walks renumber blocks: in every span of 8 * kXcdRun
+// blocks, the kXcdRun blocks an XCD receives take kXcdRun consecutive tiles. Neighbours
+// then share an L2 while all XCDs still advance through the tiles together (a whole-range
+// split per XCD leaves XCDs with unequal work). Launches use a grid that is a multiple of
+// the span (xcd_grid); any other grid keeps the hardware order.
 constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdRun = 8;
+constexpr uint32_t kXcdSpan = kXcds * kXcdRun;
 
 __device__ __forceinline__ uint32_t xcd_block() {
   const uint32_t g = gridDim.x, b = blockIdx.x;
-  return (g % kXcds) ? b : (b % kXcds) * (g / kXcds) + b / kXcds;
+  if (g % kXcdSpan) return b;
+  const uint32_t w = b % kXcdSpan;
+  return (b - w) + (w % kXcds) * kXcdRun + w / kXcds;
 }
 
 struct Acc3 {
@@ -1332,11 +1338,11 @@ static inline uint32_t grid_for(uint32_t ntiles, uint32_t cap = 65535u) {
   return ntiles == 0 ? 1u : (ntiles < cap ? ntiles : cap);
 }
 
-// Grid for the XCD-major walks: a multiple of 8 blocks (idle blocks skip the tile loop).
+// Grid for the XCD-ordered walks: a multiple of the span (idle blocks skip the tile loop).
 static inline uint32_t xcd_grid(uint32_t ntiles) {
-  if (ntiles < kXcds) return ntiles == 0 ? 1u : ntiles;
-  const uint32_t g = (ntiles + kXcds - 1) / kXcds * kXcds;
-  return g < 65528u ? g : 65528u;
+  if (ntiles < kXcdSpan) return ntiles == 0 ? 1u : ntiles;
+  const uint32_t g = (ntiles + kXcdSpan - 1) / kXcdSpan * kXcdSpan;
+  return g < 65472u ? g : 65472u;
 }
 
 hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg) {

@@ -3,7 +3,7 @@
 
 #include <mutex>
 
-#include "../../include/galaxy_lbvh.h"
+#include "galaxy_lbvh.h"
 
 namespace glx {
 

@@ -137,3 +137,17 @@ def test_pool_forces_match_oracle_per_job():
     cnt = p.count()
     want = sum(O.lbvh_accelerations(pos, mass, 1.0, 0.01, 0.5)[1] for pos, vel, mass in sy)
     assert abs(cnt["point_terms"] - want) / want < 1e-3
+
+
+def test_leaf_size_autotuner_picks_a_candidate_within_the_error_bound():
+    from galaxy_amd.tune import tune_leaf_size
+
+    sy = systems([3000, 1500, 5000, 800], seed=70)
+    r = tune_leaf_size(sy, softening=0.01, theta=0.5, steps=2, candidates=(1, 4, 16))
+    assert r["leaf_size"] in (1, 4, 16) and len(r["table"]) == 3
+    best = next(t for t in r["table"] if t["leaf_size"] == r["leaf_size"])
+    assert best["median_err"] <= 1e-3
+    assert best["rate"] == max(t["rate"] for t in r["table"] if t["median_err"] <= 1e-3)
+    # buckets are exact where leaf size 1 might accept a descendant: error never grows
+    errs = [t["median_err"] for t in r["table"]]
+    assert errs[2] <= errs[0] * 1.01
