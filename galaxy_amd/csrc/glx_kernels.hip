@@ -1093,13 +1093,6 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
         const int act = point ? 1 : (accept ? 2 : (open ? 3 : (bkt ? 4 : 0)));
         const float4 P = is_leaf ? LP : A;
-        float4 Q0 = make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 Q1 = make_float2(0.f, 0.f);
-        if (QUAD) {
-          const float4 q1 = s.quad[2 * ni + 1];
-          Q0 = s.quad[2 * ni];
-          Q1 = make_float2(q1.x, q1.y);
-        }
         const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
         const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
                                  m3 = __ballot(act == 3), m4 = __ballot(act == 4);
@@ -1109,11 +1102,12 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           break;
         }
         if (act == 1) pl[np + __popcll(m1 & lt)] = P;
-        if (QUAD && act == 2) {
+        if (QUAD && act == 2) {  // only accepted nodes fetch their moments
           const uint32_t q = nq + __popcll(m2 & lt);
+          const float4 q1 = s.quad[2 * ni + 1];
           qa[q] = P;
-          qb[q] = Q0;
-          qc[q] = Q1;
+          qb[q] = s.quad[2 * ni];
+          qc[q] = make_float2(q1.x, q1.y);
         }
         if (act == 3) {
           const uint32_t q = sp + 2u * (uint32_t)__popcll(m3 & lt);
