@@ -19,6 +19,7 @@ struct Job {
   uint64_t steps_done = 0;
   double time = 0.0;
   bool done = false;
+  bool primed = false;  // a(x0) computed (new jobs are primed together before their first step)
 };
 }  // namespace
 
@@ -48,17 +49,31 @@ struct glx_pool {
   }
 };
 
-// Rebuild the stepping batch from the active jobs (id order) and re-capture its graph.
+// Rebuild the stepping batch from the active jobs (id order), prime the jobs admitted
+// since the last step with one batched force evaluation, and re-capture the step graph.
 static void pool_refresh(glx_pool* p) {
   if (!p->dirty) return;
   p->graph.drop();
   p->active.clear();
-  std::vector<SegDesc> descs;
+  std::vector<SegDesc> descs, fresh;
+  std::vector<Job*> unprimed;
   for (auto& kv : p->jobs) {
     Job* j = kv.second.get();
+    if (!j->primed) {
+      fresh.push_back(j->seg.desc);
+      unprimed.push_back(j);
+    }
     if (j->done) continue;
     p->active.push_back(j);
     descs.push_back(j->seg.desc);
+  }
+  if (!fresh.empty()) {
+    Batch& b = *p->prime;
+    b.set_segments(fresh);
+    b.reset_bbox();
+    b.bbox();
+    b.build_and_walk(0, false);
+    for (Job* j : unprimed) j->primed = true;
   }
   if (!descs.empty()) {
     p->batch->set_segments(descs);
@@ -140,12 +155,7 @@ extern "C" int glx_pool_admit(glx_pool* p, const glx_job* job, uint64_t* id) {
     j->seg.desc.dt = job->dt;
     j->seg.desc.half_dt = (float)(0.5 * job->dt);
     upload_state(j->seg.desc, job->n, job->pos, job->vel, job->mass, p->st);
-    // prime a(x0) for the opening half-kick
-    p->prime->set_segments({j->seg.desc});
-    p->prime->reset_bbox();
-    p->prime->bbox();
-    p->prime->build_and_walk(0, false);
-    sync(p->st);
+    sync(p->st);  // a(x0) follows at the next refresh, batched with the other new jobs
     j->id = p->next_id++;
     j->steps_target = job->steps;
     p->used += j->seg.bytes();
@@ -242,6 +252,7 @@ extern "C" int glx_pool_snapshot_accel(glx_pool* p, uint64_t id, double* acc) {
   return guard([&] {
     if (!p || !acc) fail(GLX_EINVAL, "NULL argument");
     const Job* j = find_job(p, id);
+    pool_refresh(p);  // primes a job admitted since the last step
     const size_t n = j->seg.desc.n;
     std::vector<float> a32(3 * n);
     d2h(a32.data(), j->seg.desc.acc, 3 * n * sizeof(float), p->st);

@@ -252,7 +252,8 @@ typedef struct glx_pool_info {
 int glx_pool_create(const glx_params *build, uint64_t budget_bytes, uint32_t max_jobs,
                     glx_pool **out);
 int glx_pool_destroy(glx_pool *pool);
-/* Uploads the job and primes a(x0). GLX_EBUDGET / GLX_ENOSLOT when it does not fit. */
+/* Uploads the job; its a(x0) is computed before the next step, in one batched launch with
+ * the other jobs admitted since. GLX_EBUDGET / GLX_ENOSLOT when it does not fit. */
 int glx_pool_admit(glx_pool *pool, const glx_job *job, uint64_t *id);
 /* One KDK step of every active job; device_ms (may be NULL) = device time of the step. */
 int glx_pool_step(glx_pool *pool, double *device_ms);
