@@ -44,6 +44,12 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector with packed FP32; 78.6 without
 FP32_NONPACKED_TFLOPS = 78.6
 FLOPS_POINT = 20
 FLOPS_QUAD = 56
+# HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
+# `bench.py --profile --leaf-size 1` (--pmc FETCH_SIZE, then --pmc WRITE_SIZE), each scaled
+# by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
+# reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 47.93 GB fetched + 0.56 GB
+# written; the walk fetches each tested node record from beyond L2.
+WALK_TRAFFIC_BYTES = 47.93e9 + 0.557e9
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
@@ -517,7 +523,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
             "frac_vs_non_packed_peak": b["walk"]["tflops"] / FP32_NONPACKED_TFLOPS,
-            "traffic": None,
+            "traffic": WALK_TRAFFIC_BYTES,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale",
             "flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
         },
