@@ -421,6 +421,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="job arrivals per second")
     ap.add_argument("--leaf-size", type=int, default=0,
                     help="walk leaf size of the ensemble (0: autotune on 64 of its systems)")
+    ap.add_argument("--far-theta", type=float, default=-1.0,
+                    help="far-field ratio of the group walk (0: off; negative: autotune)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
     ap.add_argument("--precision-only", action="store_true", help="only the precision stress")
     ap.add_argument("--profile", action="store_true",
@@ -480,17 +482,20 @@ def main():
     log(f"ensemble: {len(systems)} systems, {int(sizes.sum())} particles, "
         f"ICs {time.perf_counter() - t0:.1f}s")
     tuning = None
-    if args.leaf_size:
+    far = max(args.far_theta, 0.0)
+    if args.leaf_size and args.far_theta >= 0:
         leaf = args.leaf_size
     else:
         from galaxy_amd.tune import tune_leaf_size
 
         pick = np.random.default_rng(args.seed + 7).choice(len(systems), 64, replace=False)
         tuning = tune_leaf_size([systems[k] for k in sorted(pick)], softening=eps, theta=theta,
-                                dt=dt)
-        leaf = tuning["leaf_size"]
-        log(f"tuned leaf size {leaf}: {tuning['table']}")
+                                dt=dt, leaf_sizes=(args.leaf_size,) if args.leaf_size else None,
+                                far_thetas=(args.far_theta,) if args.far_theta >= 0 else None)
+        leaf, far = tuning["leaf_size"], tuning["far_theta"]
+        log(f"tuned leaf size {leaf}, far_theta {far}: {tuning['table']}")
     fast_mode["leaf_size"] = leaf
+    fast_mode["far_theta"] = far
     b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, fast_mode, 8)
     log(f"batched: {b['value']:.4g} pf/s, {b['ms_per_step']:.2f} ms/step, err {b['error']}")
     out = {
@@ -511,7 +516,7 @@ def main():
                    "systems": len(systems), "particles": int(sizes.sum()),
                    "n_range": [1024, 65536], "theta": theta, "eps": eps, "dt": dt,
                    "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
-                   "leaf_size": leaf},
+                   "leaf_size": leaf, "far_theta": far},
         "p50_step_ms": b["device_ms"]["p50"],
         "p99_step_ms": b["device_ms"]["p99"],
         "force_error_vs_fp64_direct": b["error"],

@@ -54,6 +54,10 @@ inline void validate_params(const glx_params* p) {
   if (p->walk > GLX_WALK_GROUP) fail(GLX_EINVAL, "unknown walk");
   if (p->multipole == 2 && p->walk != GLX_WALK_GROUP)
     fail(GLX_EINVAL, "quadrupole moments need the group walk");
+  if (!isfinite(p->far_theta) || p->far_theta < 0.0 || p->far_theta >= 1.0)
+    fail(GLX_EINVAL, "far_theta must be in [0, 1)");
+  if (p->far_theta > 0.0 && (p->walk != GLX_WALK_GROUP || p->multipole != 2))
+    fail(GLX_EINVAL, "far_theta needs the group walk with quadrupoles");
 }
 
 inline Mode mode_from(const glx_params& p) {
@@ -62,6 +66,7 @@ inline Mode mode_from(const glx_params& p) {
   m.group = p.walk == GLX_WALK_GROUP;
   m.quad = p.multipole == 2;
   m.kahan = p.precision == GLX_PREC_KAHAN;
+  m.far = p.far_theta > 0.0;
   return m;
 }
 
@@ -76,6 +81,7 @@ inline void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
   d.theta = (float)p.theta;
   d.precision = p.precision;
   d.leaf_size = p.leaf_size ? p.leaf_size : 1u;
+  d.far2 = (float)(p.far_theta * p.far_theta);
 }
 
 inline hipStream_t make_stream() {
@@ -117,7 +123,7 @@ inline void sync(hipStream_t st) { check(hipStreamSynchronize(st), "hipStreamSyn
 inline void zero_walk_counters(const SegDesc& d, hipStream_t st) {
   check(hipMemsetAsync(d.counters + kCntInteractions, 0, 2 * sizeof(unsigned long long), st),
         "memset counters");
-  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 3 * sizeof(unsigned long long), st),
+  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 4 * sizeof(unsigned long long), st),
         "memset counters");
 }
 
@@ -128,6 +134,7 @@ inline glx_stats stats_from(const unsigned long long* c) {
   s.node_tests = c[kCntVisits];
   s.fallbacks = c[kCntOverflow];
   s.max_stack = c[kCntMaxStack];
+  s.far_nodes = c[kCntFar];
   return s;
 }
 

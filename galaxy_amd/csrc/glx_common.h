@@ -41,6 +41,7 @@ struct Mode {
   bool group = false;    // wave-cooperative group walk instead of the per-target walk
   bool quad = false;     // quadrupole moments (group walk only)
   bool kahan = false;    // Kahan-compensated accumulation
+  bool far = false;      // per-group local expansion of well-separated nodes (group walk)
 };
 
 enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1 };
@@ -53,6 +54,7 @@ enum : int {
   kCntOverflow = 3,      // group walks that fell back to the per-target walk
   kCntQuad = 4,          // quadrupole-node terms, per target
   kCntMaxStack = 5,      // deepest group-walk stack, entries
+  kCntFar = 6,           // nodes folded into a group's local expansion, per group
   kCounters = 8
 };
 
@@ -88,6 +90,7 @@ struct SegDesc {
   uint32_t precision;
   uint32_t leaf_size;  // group walk: nodes of at most this many bodies are summed body by body
   float g, eps2, theta, half_dt;
+  float far2;  // squared far-field ratio: (node radius + group radius)^2 <= far2 * R^2
   double dt;
 };
 

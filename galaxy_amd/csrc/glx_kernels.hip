@@ -990,6 +990,84 @@ __device__ __forceinline__ void quad_term(Acc3& a, const float4& me, const float
   }
 }
 
+// Far field of a group: a node (com at d from the group centre, mass m, traceless Q) is
+// folded into a second-order local expansion of the softened acceleration about the
+// centre, a(c + y) = A0 + A1 y + A2 y y / 2 (R^2 = |d|^2 + eps^2):
+//   monopole   A0 = m d / R^3,  A1_ij = m (3 d_i d_j / R^5 - delta_ij / R^3),
+//              A2_ijk = m (15 d_i d_j d_k / R^7 - 3 (delta_ij d_k + delta_ik d_j + delta_jk d_i) / R^5)
+//   quadrupole A0 = -q / R^5 + 2.5 s d / R^7 (q = Q d, s = d.q),
+//              A1_ij = Q_ij / R^5 - 5 (q_i d_j + q_j d_i) / R^7 - 2.5 s delta_ij / R^7
+//                      + 17.5 s d_i d_j / R^9
+// (the quadrupole's second-order term is dropped: it is smaller than the neglected
+// third-order monopole term for well-separated nodes). Coefficients: L[0..2] = A0,
+// L[3..8] = A1 {xx,yy,zz,xy,xz,yz}, L[9..18] = A2 {xxx,xxy,xxz,xyy,xyz,xzz,yyy,yyz,yzz,zzz}.
+constexpr int kLocal = 19;
+
+__device__ __forceinline__ void far_m2l(float* L, float x, float y, float z, float m,
+                                        const float4& Q0, const float4& Q1, float eps2) {
+  const float R2 = fmaf(x, x, fmaf(y, y, fmaf(z, z, eps2)));
+  const float ir = __builtin_amdgcn_rsqf(R2), ir2 = ir * ir;
+  const float mr3 = m * ir * ir2, ir5 = ir * ir2 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  // monopole
+  L[0] = fmaf(mr3, x, L[0]);
+  L[1] = fmaf(mr3, y, L[1]);
+  L[2] = fmaf(mr3, z, L[2]);
+  const float t = 3.0f * m * ir5;
+  L[3] += fmaf(t, xx, -mr3);
+  L[4] += fmaf(t, yy, -mr3);
+  L[5] += fmaf(t, zz, -mr3);
+  L[6] = fmaf(t, xy, L[6]);
+  L[7] = fmaf(t, xz, L[7]);
+  L[8] = fmaf(t, yz, L[8]);
+  const float w = 15.0f * m * ir7;
+  L[9] += x * fmaf(w, xx, -3.0f * t);
+  L[10] += y * fmaf(w, xx, -t);
+  L[11] += z * fmaf(w, xx, -t);
+  L[12] += x * fmaf(w, yy, -t);
+  L[13] = fmaf(w * xy, z, L[13]);
+  L[14] += x * fmaf(w, zz, -t);
+  L[15] += y * fmaf(w, yy, -3.0f * t);
+  L[16] += z * fmaf(w, yy, -t);
+  L[17] += y * fmaf(w, zz, -t);
+  L[18] += z * fmaf(w, zz, -3.0f * t);
+  // quadrupole {xx,yy,zz,xy} = Q0, {xz,yz} = Q1.xy
+  const float qx = fmaf(Q0.x, x, fmaf(Q0.w, y, Q1.x * z));
+  const float qy = fmaf(Q0.w, x, fmaf(Q0.y, y, Q1.y * z));
+  const float qz = fmaf(Q1.x, x, fmaf(Q1.y, y, Q0.z * z));
+  const float s = fmaf(x, qx, fmaf(y, qy, z * qz));
+  const float f = 2.5f * s * ir7, e = 7.0f * f * ir2, g5 = 5.0f * ir7;
+  L[0] += fmaf(f, x, -qx * ir5);
+  L[1] += fmaf(f, y, -qy * ir5);
+  L[2] += fmaf(f, z, -qz * ir5);
+  L[3] += fmaf(Q0.x, ir5, fmaf(e, xx, -f)) - 2.0f * g5 * qx * x;
+  L[4] += fmaf(Q0.y, ir5, fmaf(e, yy, -f)) - 2.0f * g5 * qy * y;
+  L[5] += fmaf(Q0.z, ir5, fmaf(e, zz, -f)) - 2.0f * g5 * qz * z;
+  L[6] += fmaf(Q0.w, ir5, e * xy) - g5 * fmaf(qx, y, qy * x);
+  L[7] += fmaf(Q1.x, ir5, e * xz) - g5 * fmaf(qx, z, qz * x);
+  L[8] += fmaf(Q1.y, ir5, e * yz) - g5 * fmaf(qy, z, qz * y);
+}
+
+// a(c + y) from the local coefficients.
+__device__ __forceinline__ void far_l2p(const float* L, float x, float y, float z, float& ax,
+                                        float& ay, float& az) {
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  ax = L[0] + fmaf(L[3], x, fmaf(L[6], y, L[7] * z)) +
+       0.5f * fmaf(L[9], xx, fmaf(L[12], yy, fmaf(L[14], zz,
+              2.0f * fmaf(L[10], xy, fmaf(L[11], xz, L[13] * yz)))));
+  ay = L[1] + fmaf(L[6], x, fmaf(L[4], y, L[8] * z)) +
+       0.5f * fmaf(L[10], xx, fmaf(L[15], yy, fmaf(L[17], zz,
+              2.0f * fmaf(L[12], xy, fmaf(L[13], xz, L[16] * yz)))));
+  az = L[2] + fmaf(L[7], x, fmaf(L[8], y, L[5] * z)) +
+       0.5f * fmaf(L[11], xx, fmaf(L[16], yy, fmaf(L[18], zz,
+              2.0f * fmaf(L[13], xy, fmaf(L[14], xz, L[17] * yz)))));
+}
+
+__device__ __forceinline__ float wave_sumf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Bodies of the buckets in `rl[0, nr)` (first leaf, count) on the lane's target. The
 // range is wave-uniform, so the body records come through scalar loads.
 template <bool KAHAN>
@@ -1006,7 +1084,7 @@ __device__ __forceinline__ uint32_t bucket_terms(Acc3& a, const float4& me, cons
   return total;
 }
 
-template <bool KAHAN, bool COUNT, bool QUAD>
+template <bool KAHAN, bool COUNT, bool QUAD, bool FAR>
 __global__ void __launch_bounds__(kGroup)
 k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
              int mode) {
@@ -1029,7 +1107,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     // that start in this chunk of sorted leaves, one after the other
     const uint32_t ge = lane < tl.count ? s.gend[tl.start + lane] : kEnd;
     unsigned long long starts = __ballot(ge != kEnd);
-    unsigned long long t_int = 0, t_vis = 0, t_quad = 0;
+    unsigned long long t_int = 0, t_vis = 0, t_quad = 0, t_far = 0;
     uint32_t t_peak = 0;
     while (starts != 0ull) {
       const uint32_t l0 = (uint32_t)__builtin_ctzll(starts);
@@ -1046,8 +1124,15 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       const float4 me = s.leaf[k];
       const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
       const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
+      // far field: group centre and radius; this lane's share of the local expansion
+      const float cx = 0.5f * (gx0 + gx1), cy = 0.5f * (gy0 + gy1), cz = 0.5f * (gz0 + gz1);
+      const float hx = 0.5f * (gx1 - gx0), hy = 0.5f * (gy1 - gy0), hz = 0.5f * (gz1 - gz0);
+      const float rg = FAR ? sqrtf(fmaf(hx, hx, fmaf(hy, hy, hz * hz))) : 0.0f;
+      float loc[FAR ? kLocal : 1];
+#pragma unroll
+      for (int c = 0; c < (FAR ? kLocal : 1); ++c) loc[c] = 0.0f;
       Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      unsigned long long n_int = 0, n_vis = 0, n_quad = 0;
+      unsigned long long n_int = 0, n_vis = 0, n_quad = 0, n_far = 0;
       uint32_t sp = 1, np = 0, nq = 0, nr = 0, peak = 1;
       bool overflow = false;
       if (lane == 0) stk[0] = (n == 1) ? 0u : n;
@@ -1091,7 +1176,16 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool bkt = near & (ncount <= bucket);
         const bool open = near & !bkt;
         const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
-        const int act = point ? 1 : (accept ? 2 : (open ? 3 : (bkt ? 4 : 0)));
+        // act 5: accepted and well separated from the whole group -> local expansion
+        bool far = false;
+        if (FAR) {
+          const float bx = fmaxf(A.x - B.x, Cc.x - A.x), by = fmaxf(A.y - B.y, Cc.y - A.y),
+                      bz = fmaxf(A.z - B.z, Cc.z - A.z);
+          const float rn = sqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz))) + rg;
+          const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
+          far = accept & (rn * rn <= s.far2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+        }
+        const int act = point ? 1 : (accept ? (far ? 5 : 2) : (open ? 3 : (bkt ? 4 : 0)));
         const float4 P = is_leaf ? LP : A;
         const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
         const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
@@ -1115,6 +1209,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           stk[q + 1] = cl;  // left child popped first
         }
         if (act == 4) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
+        if (FAR && act == 5) {
+          far_m2l(loc, A.x - cx, A.y - cy, A.z - cz, A.w, s.quad[2 * ni], s.quad[2 * ni + 1], eps2);
+          if (COUNT) ++n_far;
+        }
         np += (uint32_t)__popcll(m1);
         nq += (uint32_t)__popcll(m2);
         nr += (uint32_t)__popcll(m4);
@@ -1147,12 +1245,20 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
         if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
       }  // otherwise the lists were drained in the last round
+      if (FAR && !overflow) {
+#pragma unroll
+        for (int c = 0; c < (FAR ? kLocal : 1); ++c) loc[c] = wave_sumf(loc[c]);
+        float fx, fy, fz;
+        far_l2p(loc, me.x - cx, me.y - cy, me.z - cz, fx, fy, fz);
+        acc_add<KAHAN>(a, fx, fy, fz);
+      }
       if (valid) store_acc<KAHAN>(s, k, a, mode);
       if (COUNT) {
         // n_int is per target (uniform over valid lanes unless the group fell back)
         t_int += wave_sum_u64(valid ? n_int : 0ull);
         t_quad += wave_sum_u64(valid ? n_quad : 0ull);
         t_vis += wave_sum_u64(n_vis);
+        t_far += wave_sum_u64(n_far);
         t_peak = peak > t_peak ? peak : t_peak;
       }
       __syncthreads();
@@ -1162,6 +1268,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       atomicAdd(&s.counters[kCntQuad], t_quad);
       atomicAdd(&s.counters[kCntVisits], t_vis);
       atomicMax(&s.counters[kCntMaxStack], (unsigned long long)t_peak);
+      atomicAdd(&s.counters[kCntFar], t_far);
     }
   }
 }
@@ -1437,25 +1544,26 @@ hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs,
 }
 
 template <bool KAHAN, bool COUNT>
-static void walk_group_q(hipStream_t st, bool quad, const SegDesc* segs, const Tile* gt,
+static void walk_group_q(hipStream_t st, bool quad, bool far, const SegDesc* segs, const Tile* gt,
                          uint32_t ngt, int mode) {
-  if (quad)
-    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true>), dim3(xcd_grid(ngt)), dim3(kGroup), 0, st,
-                       segs, gt, ngt, mode);
+  const dim3 g(xcd_grid(ngt)), b(kGroup);
+  if (quad && far)
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true, true>), g, b, 0, st, segs, gt, ngt, mode);
+  else if (quad)
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true, false>), g, b, 0, st, segs, gt, ngt, mode);
   else
-    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false>), dim3(xcd_grid(ngt)), dim3(kGroup), 0,
-                       st, segs, gt, ngt, mode);
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false, false>), g, b, 0, st, segs, gt, ngt, mode);
 }
 
 hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* wt,
                        uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count) {
   if (md.group) {
     if (md.kahan) {
-      if (count) walk_group_q<true, true>(st, md.quad, segs, gt, ngt, mode);
-      else walk_group_q<true, false>(st, md.quad, segs, gt, ngt, mode);
+      if (count) walk_group_q<true, true>(st, md.quad, md.far, segs, gt, ngt, mode);
+      else walk_group_q<true, false>(st, md.quad, md.far, segs, gt, ngt, mode);
     } else {
-      if (count) walk_group_q<false, true>(st, md.quad, segs, gt, ngt, mode);
-      else walk_group_q<false, false>(st, md.quad, segs, gt, ngt, mode);
+      if (count) walk_group_q<false, true>(st, md.quad, md.far, segs, gt, ngt, mode);
+      else walk_group_q<false, false>(st, md.quad, md.far, segs, gt, ngt, mode);
     }
     return hipGetLastError();
   }

@@ -56,6 +56,11 @@ typedef struct glx_params {
   uint32_t flags;       /* GLX_FLAG_* */
   uint32_t morton_bits; /* 30 (reference, also 0) or 63 (2^21 cells per axis) */
   uint32_t walk;        /* GLX_WALK_*; quadrupoles need GLX_WALK_GROUP */
+  double far_theta;     /* 0 = off; in (0, 1) with GLX_WALK_GROUP and multipole 2: an
+                           accepted node whose radius plus the group's radius is at most
+                           far_theta times its distance from the group centre enters one
+                           second-order local expansion per group instead of a term per
+                           body */
 } glx_params;
 
 /* FP64 conservation diagnostics (core/src/diagnostics.rs:8-68). */
@@ -77,6 +82,7 @@ typedef struct glx_stats {
   uint64_t node_tests;  /* nodes tested (per-target walk: per target; group walk: per group) */
   uint64_t fallbacks;   /* group walks that fell back to the per-target walk */
   uint64_t max_stack;   /* deepest group-walk node stack, entries */
+  uint64_t far_nodes;   /* nodes folded into group local expansions (per group) */
 } glx_stats;
 
 /* Message of the last failed call on this thread ("" if none). */

@@ -74,6 +74,10 @@ def test_invalid_params_rejected_before_device():
         L.Params(1.0, 0.05, 0.5, 33, 0, 0, 0, 63, 1),  # leaf_size above 32
         L.Params(1.0, 0.05, 0.5, 1, 1, 0, 0),        # multipole
         L.Params(1.0, 0.05, 0.5, 1, 0, 7, 0),        # precision
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 63, 1, 0.4),  # far field needs quadrupoles
+        L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 0, 0.4),  # ... and the group walk
+        L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, 1.0),  # far_theta >= 1
+        L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, float("nan")),
     ):
         assert lib.glx_solver_create(C.byref(bad), C.byref(h)) == L.GLX_EINVAL
         assert lib.glx_last_error()

@@ -235,3 +235,46 @@ def test_fast_resident_roundtrip_determinism_and_exact_kdk():
     dv = np.linalg.norm(outs[0].vel - ve) / np.linalg.norm(ve - vel)
     say("fast resident dv vs exact KDK (relative to dv_total)", dv)
     assert dv < 5e-3
+
+
+FAR = dict(FAST, far_theta=0.4)
+
+
+def test_far_field_theta_zero_is_direct_sum():
+    s = glx.Lbvh(G, EPS, 1e-6, **FAR)
+    for seed in range(4):
+        pos, _, mass = I.cluster(seed, 300)
+        assert I.rms_rel_err(s.accelerations(pos, mass), O.direct_sum(pos, mass, G, EPS)) < 2e-5
+
+
+def test_far_field_error_and_work():
+    """Plummer N=16384, theta=0.5 against the FP64 direct sum. A CPU emulation of the same
+    walk in f64 gives median 1.42e-4 (no far field), 1.44e-4 (far_theta 0.4) and 2.42e-4
+    (far_theta 0.5); the far field takes 30% / 53% of the accepted nodes."""
+    pos, _, mass = I.plummer_numpy(16384, seed=9)
+    eps = 0.01
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, eps, np.arange(16384, dtype=np.uint32))
+    base = glx.Lbvh(1.0, eps, 0.5, count=True, **FAST)
+    e0 = _errs(base.accelerations(pos, mass), d64)
+    st0 = base.stats()
+    for ft, med_max, p99_max in ((0.4, 1.7e-4, 1.5e-3), (0.5, 3.0e-4, 3.0e-3)):
+        s = glx.Lbvh(1.0, eps, 0.5, count=True, **dict(FAST, far_theta=ft))
+        a = s.accelerations(pos, mass)
+        e = _errs(a, d64)
+        st = s.stats()
+        say(f"far_theta {ft}: err {e} (none: {e0}) stats {st}")
+        assert e[0] <= med_max and e[1] <= p99_max
+        assert st["far_nodes"] > 0 and st["quad_terms"] < st0["quad_terms"]
+        assert st["fallbacks"] == 0
+        assert np.array_equal(a, s.accelerations(pos, mass))  # deterministic
+
+
+def test_far_field_collision_error():
+    pos, _, mass, _ = ic.collision(1 << 18, 12)
+    rng = np.random.default_rng(2)
+    tg = np.sort(rng.choice(1 << 18, 4096, replace=False)).astype(np.uint32)
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, 0.05, tg)
+    a = glx.Lbvh(1.0, 0.05, 0.6, **dict(FAST, far_theta=0.5)).accelerations(pos, mass)
+    e = _errs(a[tg], d64)
+    say(f"collision 2^18 far_theta 0.5: {e}")
+    assert e[0] <= 5e-4

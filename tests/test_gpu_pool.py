@@ -21,6 +21,7 @@ GB = 1 << 30
 MODES = {
     "reference": dict(morton_bits=30, walk=glx.WALK_REFERENCE, multipole=0),
     "fast": dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2),
+    "far": dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2, far_theta=0.5),
 }
 
 
@@ -36,7 +37,7 @@ def systems(sizes, seed=0):
     return out
 
 
-@pytest.mark.parametrize("mode", ["reference", "fast"])
+@pytest.mark.parametrize("mode", ["reference", "fast", "far"])
 def test_one_job_pool_equals_resident_stepper(mode):
     pos, vel, mass = ic.plummer(3000, 5)
     p = Pool(GB, **MODES[mode])
@@ -51,7 +52,7 @@ def test_one_job_pool_equals_resident_stepper(mode):
     assert np.array_equal(pp, s.pos) and np.array_equal(pv, s.vel)
 
 
-@pytest.mark.parametrize("mode", ["reference", "fast"])
+@pytest.mark.parametrize("mode", ["reference", "fast", "far"])
 def test_batch_composition_invariance(mode):
     sizes = [1024, 1, 2, 5000, 777, 2049, 3000, 64]
     sy = systems(sizes, seed=10)
