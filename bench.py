@@ -44,6 +44,10 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector with packed FP32; 78.6 without
 FP32_NONPACKED_TFLOPS = 78.6
 FLOPS_POINT = 20
 FLOPS_QUAD = 56
+# far field: one node folded into a group's local expansion (95 VALU operations, FMAs
+# counted twice) and the evaluation of the expansion at one body
+FLOPS_M2L = 150
+FLOPS_L2P = 60
 # HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
 # `bench.py --profile --leaf-size 1` (--pmc FETCH_SIZE, then --pmc WRITE_SIZE), each scaled
 # by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
@@ -180,7 +184,8 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
         tg = np.sort(rng.choice(m.size, min(m.size, err_per_system), replace=False))
         errs.append(rel_err(acc[tg], glx.direct_sum_f64(pos, m, 1.0, eps, tg.astype(np.uint32))))
     walk_ms = phases["walk"]
-    flops = FLOPS_POINT * cnt["point_terms"] + FLOPS_QUAD * cnt["quad_terms"]
+    flops = (FLOPS_POINT * cnt["point_terms"] + FLOPS_QUAD * cnt["quad_terms"] +
+             FLOPS_M2L * cnt["far_nodes"] + (FLOPS_L2P * n_tot if mode.get("far_theta") else 0))
     pool.close()
     return {
         "value": n_tot * steps / wall,
@@ -194,6 +199,7 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
                  "point_terms_per_particle": cnt["point_terms"] / n_tot,
                  "quad_terms_per_particle": cnt["quad_terms"] / n_tot,
                  "node_tests_per_particle": cnt["node_tests"] / n_tot,
+                 "far_nodes_per_particle": cnt["far_nodes"] / n_tot,
                  "fallbacks": cnt["fallbacks"], "max_stack": cnt["max_stack"]},
         "error": quantiles(np.concatenate(errs)),
         "admit_s": t_admit,
@@ -235,7 +241,8 @@ def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets
     graph_ms = (time.perf_counter() - g0) * 1e3 / steps
     d64 = glx.direct_sum_f64(snap.pos, mass, 1.0, eps, d64_targets)
     walk_ms = phases["walk"] / steps
-    flops = FLOPS_POINT * c0["point_terms"] + FLOPS_QUAD * c0["quad_terms"]
+    flops = (FLOPS_POINT * c0["point_terms"] + FLOPS_QUAD * c0["quad_terms"] +
+             FLOPS_M2L * c0["far_nodes"] + (FLOPS_L2P * n if mode.get("far_theta") else 0))
     bytes_tab = PHASE_BYTES_63 if mode.get("morton_bits") == 63 else PHASE_BYTES_30
     if mode.get("multipole") != 2:
         bytes_tab = dict(bytes_tab, aggregate=168)
@@ -468,6 +475,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.profile_collision:
+        fast_mode["far_theta"] = max(args.far_theta, 0.0)
         n = args.collision_n
         pos, vel, mass, _ = ic.collision(n, args.seed)
         tg = np.arange(min(n, 256), dtype=np.uint32)

@@ -937,9 +937,10 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
 // evaluate against their own targets (LDS broadcast reads, no divergence). Opened
 // nodes push both children. If the stack would overflow, the group falls back to
 // the per-target walk.
-constexpr int kStackCap = 1024;
+constexpr int kStackCap = 896;  // the deepest stack measured is 520 entries
 constexpr int kListCap = 64;   // lists are drained every round, which adds at most 64
 constexpr int kRangeCap = 64;
+constexpr int kFarCap = 128;  // far nodes are drained 64 at a time, one per lane
 
 // Floor on the softened r^2 of a point term: a source at the target's position (dx = 0
 // exactly, the target itself included) then adds 0 * finite rather than 0 * inf when the
@@ -1063,6 +1064,18 @@ __device__ __forceinline__ void far_l2p(const float* L, float x, float y, float 
               2.0f * fmaf(L[13], xy, fmaf(L[14], xz, L[17] * yz)))));
 }
 
+// The queued far nodes (node indices) into this lane's share of the local expansion:
+// lane e takes entries e, e + 64, ...
+__device__ __forceinline__ void far_drain(float* L, const SegDesc& s, const uint32_t* fl,
+                                          uint32_t nf, uint32_t lane, float cx, float cy,
+                                          float cz, float eps2) {
+  for (uint32_t e = lane; e < nf; e += kGroup) {
+    const uint32_t j = fl[e];
+    const float4 A = s.node[j].a;
+    far_m2l(L, A.x - cx, A.y - cy, A.z - cz, A.w, s.quad[2 * j], s.quad[2 * j + 1], eps2);
+  }
+}
+
 __device__ __forceinline__ float wave_sumf(float v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -1094,6 +1107,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
   __shared__ float4 qa[QN], qb[QN];
   __shared__ float2 qc[QN];
   __shared__ uint2 rl[kRangeCap];
+  __shared__ uint32_t fl[FAR ? kFarCap : 1];
   const uint32_t lane = threadIdx.x;
   const unsigned long long lt = (1ull << lane) - 1ull;
   for (uint32_t t = xcd_block(); t < ntiles; t += gridDim.x) {
@@ -1133,7 +1147,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       for (int c = 0; c < (FAR ? kLocal : 1); ++c) loc[c] = 0.0f;
       Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       unsigned long long n_int = 0, n_vis = 0, n_quad = 0, n_far = 0;
-      uint32_t sp = 1, np = 0, nq = 0, nr = 0, peak = 1;
+      uint32_t sp = 1, np = 0, nq = 0, nr = 0, nf = 0, peak = 1;
       bool overflow = false;
       if (lane == 0) stk[0] = (n == 1) ? 0u : n;
       __syncthreads();
@@ -1209,9 +1223,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           stk[q + 1] = cl;  // left child popped first
         }
         if (act == 4) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
-        if (FAR && act == 5) {
-          far_m2l(loc, A.x - cx, A.y - cy, A.z - cz, A.w, s.quad[2 * ni], s.quad[2 * ni + 1], eps2);
-          if (COUNT) ++n_far;
+        if (FAR) {
+          const unsigned long long m5 = __ballot(act == 5);
+          if (act == 5) fl[nf + __popcll(m5 & lt)] = ni;
+          nf += (uint32_t)__popcll(m5);
         }
         np += (uint32_t)__popcll(m1);
         nq += (uint32_t)__popcll(m2);
@@ -1236,6 +1251,11 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           if (COUNT) n_quad += nq;
           nq = 0;
         }
+        if (FAR && nf >= 64u) {  // one far node per lane
+          far_drain(loc, s, fl, nf, lane, cx, cy, cz, eps2);
+          if (COUNT && lane == 0) n_far += nf;
+          nf = 0;
+        }
         __syncthreads();
       }
       if (overflow) {
@@ -1246,6 +1266,8 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
       }  // otherwise the lists were drained in the last round
       if (FAR && !overflow) {
+        far_drain(loc, s, fl, nf, lane, cx, cy, cz, eps2);
+        if (COUNT && lane == 0) n_far += nf;
 #pragma unroll
         for (int c = 0; c < (FAR ? kLocal : 1); ++c) loc[c] = wave_sumf(loc[c]);
         float fx, fy, fz;
