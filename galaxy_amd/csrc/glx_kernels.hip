@@ -561,6 +561,13 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 //    Each completed node is written once as its walk record (and quadrupole).
 // ----------------------------------------------------------------------------------
 
+// A pointer from a SegDesc as a global-memory pointer: its loads become global_load
+// (counted by vmcnt only) instead of flat loads, which also hold up every LDS wait.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 struct AggV {
   float3 com, mn, mx;
   float mass;
@@ -614,10 +621,12 @@ __device__ __forceinline__ AggV fold_agg(const AggV& l, const AggV& r) {
 // Walk record of internal node j from its aggregate (solvers/src/lbvh.rs:602-611 for
 // centre/half/delta; the acceptance radius follows from :511-514:
 // theta*(d - delta) >= s  <=>  d >= s/theta + delta).
+// The walk record of internal node j (topology fields given: leaf range [lo, hi],
+// children, and the skip pointer nx = rs[hi] or kEnd).
 template <bool QUAD>
 __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const AggV& r,
-                                           float inv_theta) {
-  const uint32_t n = s.n;
+                                           float inv_theta, uint32_t lo, uint32_t hi,
+                                           uint32_t cl, uint32_t cr, uint32_t nx) {
   const float cx = (r.mn.x + r.mx.x) * 0.5f, cy = (r.mn.y + r.mx.y) * 0.5f,
               cz = (r.mn.z + r.mx.z) * 0.5f;
   const float hx = (r.mx.x - r.mn.x) * 0.5f, hy = (r.mx.y - r.mn.y) * 0.5f,
@@ -626,17 +635,24 @@ __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const A
   const float ddx = r.com.x - cx, ddy = r.com.y - cy, ddz = r.com.z - cz;
   const float delta = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
   const float rcrit = size * inv_theta + delta;
-  const uint32_t h = s.hi[j];
   Node nd;
   nd.a = make_float4(r.com.x, r.com.y, r.com.z, r.mass);
   nd.b = make_float4(r.mn.x, r.mn.y, r.mn.z, rcrit * rcrit);
-  nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(s.left[j]));
-  nd.d = make_uint4(h == n - 1 ? kEnd : s.rs[h], s.right[j], s.lo[j], h);
+  nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(cl));
+  nd.d = make_uint4(nx, cr, lo, hi);
   s.node[j] = nd;
   if (QUAD) {
     s.quad[2 * j] = make_float4(r.q[0], r.q[1], r.q[2], r.q[3]);
     s.quad[2 * j + 1] = make_float4(r.q[4], r.q[5], 0.f, 0.f);
   }
+}
+
+template <bool QUAD>
+__device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const AggV& r,
+                                           float inv_theta) {
+  const uint32_t h = s.hi[j];
+  store_node<QUAD>(s, j, r, inv_theta, s.lo[j], h, s.left[j], s.right[j],
+                   h == s.n - 1 ? kEnd : s.rs[h]);
 }
 
 template <bool QUAD>
@@ -683,25 +699,35 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
     for (uint32_t e = threadIdx.x; e < tl.count; e += kAggThreads) {
       const uint32_t k = t0 + e;
       uint32_t u = k;
-      uint32_t p = s.parent_leaf[k];
+      uint32_t p = gbl(s.parent_leaf)[k];
       // each ancestor is visited once; the bound only guards corrupted input
       for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {
         const uint32_t j = p - n;
         if (j >= n - 1) break;
-        if (s.lo[j] < t0 || s.hi[j] >= t1) {
+        // all topology of node j in one batch of independent loads
+        const uint32_t lo = gbl(s.lo)[j], hi = gbl(s.hi)[j], cl = gbl(s.left)[j];
+        const uint32_t cr = gbl(s.right)[j], pp = gbl(s.parent_int)[j];
+        if (lo < t0 || hi >= t1) {
           const unsigned long long slot = atomicAdd(&s.counters[kCntRoots], 1ull);
           if (slot < n) s.roots[slot] = u;
           break;
         }
         const uint32_t q = j - t0;
+        // Siblings hand over their aggregates through LDS only (the node records written
+        // to global memory are not read in this pass), so the release/acquire pair is
+        // limited to LDS and does not wait for the global stores.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         const uint32_t old =
-            __hip_atomic_fetch_add(&s_flag[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&s_flag[q], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         if (old == 0u) break;
+        const uint32_t nx = hi == n - 1 ? kEnd : gbl(s.rs)[hi];
         AggV ch[2];
-        const uint32_t cs[2] = {s.left[j], s.right[j]};
+        const uint32_t cs[2] = {cl, cr};
         for (int c = 0; c < 2; ++c) {
           if (cs[c] < n) {
-            ch[c] = leaf_agg(s.leaf[cs[c]]);
+            const auto* lp = gbl(reinterpret_cast<const float*>(s.leaf)) + 4 * (size_t)cs[c];
+            ch[c] = leaf_agg(make_float4(lp[0], lp[1], lp[2], lp[3]));
           } else {
             const uint32_t cq = cs[c] - n - t0;
             AggV& r = ch[c];
@@ -719,9 +745,9 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
         s_mx[0][q] = r.mx.x; s_mx[1][q] = r.mx.y; s_mx[2][q] = r.mx.z;
         if (QUAD)
           for (int w = 0; w < 6; ++w) s_q[QUAD ? w : 0][q] = r.q[w];
-        store_node<QUAD>(s, j, r, inv_theta);
+        store_node<QUAD>(s, j, r, inv_theta, lo, hi, cl, cr, nx);
         u = p;
-        p = s.parent_int[j];
+        p = pp;
       }
     }
     __syncthreads();
