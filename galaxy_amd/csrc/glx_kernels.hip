@@ -787,6 +787,9 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
           p = kNoParent;
           continue;
         }
+        // topology comes from the tree build (an earlier kernel): load it with the arrival
+        const uint32_t lo = gbl(s.lo)[j], hi = gbl(s.hi)[j], cl = gbl(s.left)[j];
+        const uint32_t cr = gbl(s.right)[j], pp = gbl(s.parent_int)[j];
         const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0u) {
@@ -794,11 +797,11 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
           continue;
         }
         __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, s.left[j]),
-                                      global_child<QUAD>(s, s.right[j]));
-        store_node<QUAD>(s, j, a, inv_theta);
+        const uint32_t nx = hi == n - 1 ? kEnd : gbl(s.rs)[hi];
+        const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, cl), global_child<QUAD>(s, cr));
+        store_node<QUAD>(s, j, a, inv_theta, lo, hi, cl, cr, nx);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        p = s.parent_int[j];
+        p = pp;
       }
     }
     __syncthreads();
