@@ -49,11 +49,11 @@ FLOPS_QUAD = 56
 FLOPS_M2L = 150
 FLOPS_L2P = 60
 # HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
-# `bench.py --profile --leaf-size 1` (--pmc FETCH_SIZE, then --pmc WRITE_SIZE), each scaled
+# `bench.py --profile --leaf-size 1 --far-theta 0.5` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled
 # by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
-# reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 47.93 GB fetched + 0.56 GB
-# written; the walk fetches each tested node record from beyond L2.
-WALK_TRAFFIC_BYTES = 47.93e9 + 0.557e9
+# reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
+# written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
+WALK_TRAFFIC_BYTES = 51.06e9 + 0.557e9
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
