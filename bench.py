@@ -536,8 +536,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
             "frac_vs_non_packed_peak": b["walk"]["tflops"] / FP32_NONPACKED_TFLOPS,
-            "traffic": WALK_TRAFFIC_BYTES,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale",
+            "traffic": WALK_TRAFFIC_BYTES if (leaf, far) == (1, 0.5) else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
+                              " leaf size 1, far-field ratio 0.5",
             "flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
         },
