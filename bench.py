@@ -309,13 +309,40 @@ def precision(glx, ic, n, seed, steps, sample=65536, eps=0.02, theta=0.5, dt=2e-
     return res
 
 
-def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nmax=2097152):
+def ic_timing(ic, n, seed, repeats=3):
+    """Host generation (glx_ic_collision, up to 16 threads) against device generation
+    (glx_ic_generate_device, which returns once the arrays are written) of the same
+    n-body collision; device time is the median of `repeats` calls into buffers allocated
+    beforehand, after one warm-up call."""
+    t0 = time.perf_counter()
+    ic.collision(n, seed)
+    host_s = time.perf_counter() - t0
+    sp = ic.spec_collision(n, seed)
+    bufs = ic.generate_device(sp)
+    lib = ic._bind()
+    dev = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        rc = lib.glx_ic_generate_device(ic.C.addressof(sp), bufs[0].ptr, bufs[1].ptr, bufs[2].ptr)
+        dev.append(time.perf_counter() - t0)
+        if rc:
+            raise RuntimeError(f"glx_ic_generate_device failed ({rc})")
+    for b in bufs:
+        b.free()
+    d = float(np.median(dev))
+    return {"n": n, "host_s": host_s, "host_threads": min(16, os.cpu_count() or 1),
+            "device_ms": 1e3 * d, "host_over_device": host_s / d}
+
+
+def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nmax=2097152,
+          device_ic=False):
     """Serving stream: collision jobs arrive as a Poisson process at `rate` per second
     (N log-uniform in [nmin, nmax], theta uniform in {0.4, 0.5, 0.7}, steps uniform in
     [50, 500], eps = 0.05, dt = 5e-3); a pool with a device-memory budget admits them in
     arrival order as memory frees up and advances every active job by one KDK step per
     iteration; finished jobs are retired. Initial conditions are made by a client thread
-    ahead of their arrival time (outside the server's clock when it keeps up)."""
+    ahead of their arrival time (outside the server's clock when it keeps up), or, with
+    device_ic, drawn on the device at admission (glx_pool_admit_ic, inside the clock)."""
     import queue
     import threading
 
@@ -341,7 +368,9 @@ def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nm
                     pass
 
     th = threading.Thread(target=client, daemon=True)
-    th.start()
+    pending = list(range(njobs)) if device_ic else []  # arrivals not yet in `waiting`
+    if not device_ic:
+        th.start()
     pool = Pool(budget, max_jobs=4096, morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
     waiting, active = [], {}
     lat, pf, wall_steps = [], 0, 0.0
@@ -354,6 +383,8 @@ def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nm
             break
         while not ready.empty():
             waiting.append(ready.get())
+        while pending and arrivals[pending[0]] <= now:
+            waiting.append((pending.pop(0), None, None, None))
         waiting.sort(key=lambda w: w[0])
         still = []
         for w in waiting:
@@ -361,13 +392,18 @@ def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nm
             if arrivals[k] > now:
                 still.append(w)
                 continue
-            jid = pool.admit(w[1], w[2], w[3], softening=0.05, theta=float(thetas[k]), dt=5e-3,
-                             steps=int(nsteps[k]))
+            if device_ic:
+                jid = pool.admit_ic(ic.spec_collision(int(sizes[k]), seed + 1000 + k),
+                                    softening=0.05, theta=float(thetas[k]), dt=5e-3,
+                                    steps=int(nsteps[k]))
+            else:
+                jid = pool.admit(w[1], w[2], w[3], softening=0.05, theta=float(thetas[k]),
+                                 dt=5e-3, steps=int(nsteps[k]))
             if jid is None:
                 rejected_wait += 1
                 still.append(w)
                 continue
-            active[jid] = w[3].size
+            active[jid] = int(sizes[k])
             admitted += 1
             queue_delay.append(time.perf_counter() - t_start - arrivals[k])
         waiting = still
@@ -386,12 +422,14 @@ def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nm
                 finished += 1
     elapsed = time.perf_counter() - t_start
     stop.set()
-    th.join(timeout=30)
+    if th.is_alive():
+        th.join(timeout=30)
     st = pool.status()
     pool.close()
     lat = np.array(lat) if lat else np.zeros(1)
     return {
         "seconds": elapsed, "rate_per_s": rate, "budget_bytes": budget,
+        "initial_conditions": "device, at admission" if device_ic else "host thread, ahead",
         "jobs_admitted": admitted, "jobs_finished": finished, "jobs_active_at_end": st["active"],
         "steps": int(lat.size),
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
@@ -431,6 +469,8 @@ def main():
     ap.add_argument("--far-theta", type=float, default=-1.0,
                     help="far-field ratio of the group walk (0: off; negative: autotune)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--serve-device-ic", action="store_true",
+                    help="serving jobs drawn on the device at admission (glx_pool_admit_ic)")
     ap.add_argument("--precision-only", action="store_true", help="only the precision stress")
     ap.add_argument("--profile", action="store_true",
                     help="pool steps only (for profilers): no plain, collision or CPU legs")
@@ -467,7 +507,7 @@ def main():
         out = {}
         if args.serve_only:
             out["serving"] = serve(glx, ic, Pool, args.serve_seconds or 60.0, args.serve_rate,
-                                   args.seed)
+                                   args.seed, device_ic=args.serve_device_ic)
         if args.precision_only:
             out["precision_stress"] = precision(glx, ic, args.precision_n or (1 << 23), args.seed,
                                                 args.precision_steps, diag_every=100)
@@ -573,8 +613,11 @@ def main():
         if not args.no_cpu:
             out["collision_2p20"]["cpu_baseline"] = cpu_collision(
                 r["snap"].pos, mass, ce, ct, r["acc0"], args.cpu_budget)
+        out["collision_2p20"]["initial_conditions"] = ic_timing(ic, n, args.seed)
+        log(f"collision ICs: {out['collision_2p20']['initial_conditions']}")
     if args.serve_seconds > 0:
-        out["serving"] = serve(glx, ic, Pool, args.serve_seconds, args.serve_rate, args.seed)
+        out["serving"] = serve(glx, ic, Pool, args.serve_seconds, args.serve_rate, args.seed,
+                               device_ic=args.serve_device_ic)
         log(f"serving: {out['serving']}")
     if args.precision_n:
         out["precision_stress"] = precision(glx, ic, args.precision_n, args.seed,

@@ -4,6 +4,7 @@
 
 #include "galaxy_sim.h"
 #include "glx_abi_util.h"
+#include "glx_ic_device.h"
 #include "glx_snapshot.h"
 
 using namespace glx;
@@ -25,6 +26,32 @@ extern "C" int glx_device_info(char* name, size_t name_len, int* compute_units,
     }
     if (compute_units) *compute_units = prop.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = (uint64_t)prop.totalGlobalMem;
+  });
+}
+
+extern "C" int glx_device_malloc(size_t bytes, void** ptr) {
+  return guard([&] {
+    if (!ptr) fail(GLX_EINVAL, "ptr is NULL");
+    *ptr = nullptr;
+    ensure_device();
+    if (bytes == 0) return;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+      *ptr = nullptr;
+      fail(GLX_ENOMEM, "hipMalloc failed");
+    }
+  });
+}
+
+extern "C" int glx_device_free(void* ptr) {
+  return guard([&] {
+    if (ptr) check(hipFree(ptr), "hipFree");
+  });
+}
+
+extern "C" int glx_device_to_host(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes && (!dst || !src)) fail(GLX_EINVAL, "NULL pointer");
+    if (bytes) check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
   });
 }
 
@@ -502,29 +529,45 @@ extern "C" int glx_sys_destroy(glx_sys* s) {
   return guard([&] { delete s; });
 }
 
+// Replace the system's state with n particles written by load(desc) on the system's
+// stream, prime a(x0) and reset the time.
+template <class Load>
+static void sys_load(glx_sys* s, size_t n, Load&& load) {
+  s->graph.drop();
+  s->time = 0.0;
+  s->n = 0;
+  s->cur_dt = NAN;
+  if (n == 0) return;
+  s->seg.reserve((uint32_t)n, false);
+  fill_params(s->seg.desc, s->p, (uint32_t)n);
+  s->seg.desc.dt = 0.0;
+  s->seg.desc.half_dt = 0.0f;
+  s->batch->set_segments({s->seg.desc});
+  load(s->seg.desc);
+  Batch& b = *s->batch;
+  b.reset_bbox();
+  b.bbox();
+  b.build_and_walk(0, false);  // prime a(x0)
+  sync(s->st);
+  s->n = (uint32_t)n;
+}
+
 extern "C" int glx_sys_upload(glx_sys* s, size_t n, const double* pos, const double* vel,
                               const double* mass) {
   return guard([&] {
     if (!s) fail(GLX_EINVAL, "sys is NULL");
     validate_n(n);
-    s->graph.drop();
-    s->time = 0.0;
-    s->n = 0;
-    s->cur_dt = NAN;
-    if (n == 0) return;
-    if (!pos || !vel || !mass) fail(GLX_EINVAL, "NULL array");
-    s->seg.reserve((uint32_t)n, false);
-    fill_params(s->seg.desc, s->p, (uint32_t)n);
-    s->seg.desc.dt = 0.0;
-    s->seg.desc.half_dt = 0.0f;
-    s->batch->set_segments({s->seg.desc});
-    upload_state(s->seg.desc, n, pos, vel, mass, s->st);
-    Batch& b = *s->batch;
-    b.reset_bbox();
-    b.bbox();
-    b.build_and_walk(0, false);  // prime a(x0)
-    sync(s->st);
-    s->n = (uint32_t)n;
+    if (n && (!pos || !vel || !mass)) fail(GLX_EINVAL, "NULL array");
+    sys_load(s, n, [&](const SegDesc& d) { upload_state(d, n, pos, vel, mass, s->st); });
+  });
+}
+
+extern "C" int glx_sys_generate(glx_sys* s, const glx_ic_spec* spec) {
+  return guard([&] {
+    if (!s || !spec) fail(GLX_EINVAL, "NULL argument");
+    ic::validate_spec(*spec);
+    sys_load(s, spec->n,
+             [&](const SegDesc& d) { ic::generate_resident(*spec, d.x, d.v, d.m, s->st); });
   });
 }
 
@@ -613,6 +656,15 @@ extern "C" int glx_sys_snapshot_accel(glx_sys* s, double* acc) {
     d2h(a32.data(), s->seg.desc.acc, 3 * n * sizeof(float), s->st);
     sync(s->st);
     for (size_t q = 0; q < 3 * n; ++q) acc[q] = (double)a32[q];
+  });
+}
+
+extern "C" int glx_sys_snapshot_mass(glx_sys* s, double* mass) {
+  return guard([&] {
+    if (!s || !mass) fail(GLX_EINVAL, "NULL argument");
+    if (s->n == 0) return;
+    d2h(mass, s->seg.desc.m, (size_t)s->n * sizeof(double), s->st);
+    sync(s->st);
   });
 }
 

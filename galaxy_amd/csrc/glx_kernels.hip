@@ -1037,7 +1037,7 @@ __device__ __forceinline__ void far_m2l(float* L, float x, float y, float z, flo
                                         const float4& Q0, const float4& Q1, float eps2) {
   const float R2 = fmaf(x, x, fmaf(y, y, fmaf(z, z, eps2)));
   const float ir = __builtin_amdgcn_rsqf(R2), ir2 = ir * ir;
-  const float mr3 = m * ir * ir2, ir5 = ir * ir2 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+  const float mr3 = m * ir * ir2, ir5 = ir * ir2 * ir2, ir7 = ir5 * ir2;
   const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
   // monopole
   L[0] = fmaf(mr3, x, L[0]);

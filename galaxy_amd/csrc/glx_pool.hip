@@ -7,6 +7,7 @@
 #include <memory>
 
 #include "glx_abi_util.h"
+#include "glx_ic_device.h"
 
 using namespace glx;
 using namespace glx::abi;
@@ -133,6 +134,35 @@ extern "C" int glx_pool_destroy(glx_pool* p) {
   return guard([&] { delete p; });
 }
 
+// Admission shared by uploaded and generated jobs: checks, reserves, fills the state
+// with load(desc) on the pool stream, and queues the job for priming.
+template <class Load>
+static void admit_job(glx_pool* p, size_t n, double g, double softening, double theta,
+                      double dt, uint64_t steps, uint64_t* id, Load&& load) {
+  glx_params jp = p->build;
+  jp.g = g;
+  jp.softening = softening;
+  jp.theta = theta;
+  validate_params(&jp);
+  if (!isfinite(dt)) fail(GLX_EINVAL, "dt must be finite");
+  if (p->jobs.size() >= p->max_jobs) fail(GLX_ENOSLOT, "no free job slot");
+  const uint64_t need = Segment::bytes_for((uint32_t)n, false);
+  if (p->used + need > p->budget) fail(GLX_EBUDGET, "device-memory budget exhausted");
+  auto j = std::make_unique<Job>();
+  j->seg.reserve((uint32_t)n, false);
+  fill_params(j->seg.desc, jp, (uint32_t)n);
+  j->seg.desc.dt = dt;
+  j->seg.desc.half_dt = (float)(0.5 * dt);
+  load(j->seg.desc);
+  sync(p->st);  // a(x0) follows at the next refresh, batched with the other new jobs
+  j->id = p->next_id++;
+  j->steps_target = steps;
+  p->used += j->seg.bytes();
+  *id = j->id;
+  p->jobs.emplace(j->id, std::move(j));
+  p->dirty = true;
+}
+
 extern "C" int glx_pool_admit(glx_pool* p, const glx_job* job, uint64_t* id) {
   return guard([&] {
     if (!p || !job || !id) fail(GLX_EINVAL, "NULL argument");
@@ -140,28 +170,19 @@ extern "C" int glx_pool_admit(glx_pool* p, const glx_job* job, uint64_t* id) {
     validate_n(job->n);
     if (job->n == 0) fail(GLX_EINVAL, "a job needs at least one particle");
     if (!job->pos || !job->vel || !job->mass) fail(GLX_EINVAL, "NULL array");
-    glx_params jp = p->build;
-    jp.g = job->g;
-    jp.softening = job->softening;
-    jp.theta = job->theta;
-    validate_params(&jp);
-    if (!isfinite(job->dt)) fail(GLX_EINVAL, "dt must be finite");
-    if (p->jobs.size() >= p->max_jobs) fail(GLX_ENOSLOT, "no free job slot");
-    const uint64_t need = Segment::bytes_for((uint32_t)job->n, false);
-    if (p->used + need > p->budget) fail(GLX_EBUDGET, "device-memory budget exhausted");
-    auto j = std::make_unique<Job>();
-    j->seg.reserve((uint32_t)job->n, false);
-    fill_params(j->seg.desc, jp, (uint32_t)job->n);
-    j->seg.desc.dt = job->dt;
-    j->seg.desc.half_dt = (float)(0.5 * job->dt);
-    upload_state(j->seg.desc, job->n, job->pos, job->vel, job->mass, p->st);
-    sync(p->st);  // a(x0) follows at the next refresh, batched with the other new jobs
-    j->id = p->next_id++;
-    j->steps_target = job->steps;
-    p->used += j->seg.bytes();
-    *id = j->id;
-    p->jobs.emplace(j->id, std::move(j));
-    p->dirty = true;
+    admit_job(p, job->n, job->g, job->softening, job->theta, job->dt, job->steps, id,
+              [&](const SegDesc& d) { upload_state(d, job->n, job->pos, job->vel, job->mass, p->st); });
+  });
+}
+
+extern "C" int glx_pool_admit_ic(glx_pool* p, const glx_ic_spec* spec, double softening,
+                                 double theta, double dt, uint64_t steps, uint64_t* id) {
+  return guard([&] {
+    if (!p || !spec || !id) fail(GLX_EINVAL, "NULL argument");
+    *id = 0;
+    ic::validate_spec(*spec);
+    admit_job(p, spec->n, ic::spec_g(*spec), softening, theta, dt, steps, id,
+              [&](const SegDesc& d) { ic::generate_resident(*spec, d.x, d.v, d.m, p->st); });
   });
 }
 

@@ -92,6 +92,9 @@ def lib():
     sig = {
         "glx_last_error": (C.c_char_p, []),
         "glx_device_info": (I, [C.c_char_p, S, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+        "glx_device_malloc": (I, [S, C.POINTER(P)]),
+        "glx_device_free": (I, [P]),
+        "glx_device_to_host": (I, [P, P, S]),
         "glx_solver_create": (I, [C.POINTER(Params), C.POINTER(P)]),
         "glx_solver_destroy": (I, [P]),
         "glx_solver_accelerations": (I, [P, S, P, P, P]),
@@ -111,6 +114,8 @@ def lib():
         "glx_sys_create": (I, [C.POINTER(Params), C.POINTER(P)]),
         "glx_sys_destroy": (I, [P]),
         "glx_sys_upload": (I, [P, S, P, P, P]),
+        "glx_sys_generate": (I, [P, P]),
+        "glx_sys_snapshot_mass": (I, [P, P]),
         "glx_sys_step": (I, [P, D, C.c_uint64]),
         "glx_sys_snapshot": (I, [P, P, P]),
         "glx_sys_step_timed": (I, [P, D, C.c_uint64, P]),
@@ -386,6 +391,13 @@ class GpuResidentLeapfrog:
             raise ValueError("pos/vel/mass lengths differ")
         self._mass = mass.copy()
         _check(lib().glx_sys_upload(self._h, n, _p(pos), _p(vel), _p(mass)))
+
+    def generate(self, spec):
+        """upload(sample(...)) with the state drawn on the device (`galaxy_amd.ic.Spec`,
+        e.g. `ic.spec_collision(n, seed)`): no host arrays, no upload."""
+        _check(lib().glx_sys_generate(self._h, C.addressof(spec)))
+        self._mass = np.zeros(len(self))
+        _check(lib().glx_sys_snapshot_mass(self._h, _p(self._mass)))
 
     def step(self, dt: float):
         _check(lib().glx_sys_step(self._h, float(dt), 1))

@@ -61,6 +61,7 @@ def _bind():
         "glx_pool_create": (I, [P, C.c_uint64, C.c_uint32, C.POINTER(P)]),
         "glx_pool_destroy": (I, [P]),
         "glx_pool_admit": (I, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+        "glx_pool_admit_ic": (I, [P, P, D, D, D, C.c_uint64, C.POINTER(C.c_uint64)]),
         "glx_pool_step": (I, [P, C.POINTER(D)]),
         "glx_pool_step_timed": (I, [P, P]),
         "glx_pool_count": (I, [P, C.POINTER(Stats)]),
@@ -125,6 +126,19 @@ class Pool:
             return None
         _check(rc)
         self._n[jid.value] = mass.size
+        return jid.value
+
+    def admit_ic(self, spec, *, softening=0.05, theta=0.5, dt=1e-3, steps=0):
+        """Admit a job whose state is drawn on the device by a generator
+        (`galaxy_amd.ic.Spec`); returns its id, or None when the budget or the slots are
+        full."""
+        jid = C.c_uint64(0)
+        rc = _bind().glx_pool_admit_ic(self._h, C.addressof(spec), float(softening),
+                                       float(theta), float(dt), int(steps), C.byref(jid))
+        if rc in (GLX_EBUDGET, GLX_ENOSLOT):
+            return None
+        _check(rc)
+        self._n[jid.value] = int(spec.n)
         return jid.value
 
     def step(self) -> float:

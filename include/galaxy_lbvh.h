@@ -91,6 +91,13 @@ const char *glx_last_error(void);
 /* Device name, compute-unit count and HBM bytes of device 0 (GLX_EDEVICE if none). */
 int glx_device_info(char *name, size_t name_len, int *compute_units, uint64_t *hbm_bytes);
 
+/* Device memory for callers without an allocator of their own (the Python front end uses
+ * them for glx_ic_generate_device): hipMalloc / hipFree / a synchronous device-to-host
+ * copy. No reference counterpart. */
+int glx_device_malloc(size_t bytes, void **ptr);
+int glx_device_free(void *ptr);
+int glx_device_to_host(void *dst, const void *src, size_t bytes);
+
 /* ---------------------------------------------------------------------------------
  * ForceSolver (core/src/traits.rs:24-27,52-54,95) as implemented by Lbvh
  * (solvers/src/lbvh.rs:643-683) and GpuLbvhFused (gpu/src/gpu_lbvh_fused.rs:50-134).
@@ -190,6 +197,7 @@ int glx_stage_aggregate(size_t n, const uint32_t *sorted_codes, const double *so
  * launch. Positions are FP64 on the device, velocities and forces FP32.
  * ------------------------------------------------------------------------------- */
 typedef struct glx_sys glx_sys;
+struct glx_ic_spec; /* galaxy_ic.h */
 
 /* GpuResidentLeapfrog::new (gpu/src/gpu_resident.rs:1292). */
 int glx_sys_create(const glx_params *params, glx_sys **out);
@@ -197,6 +205,11 @@ int glx_sys_destroy(glx_sys *sys);
 /* upload (gpu/src/gpu_resident.rs:1685): copies state, primes a(x0), resets time. */
 int glx_sys_upload(glx_sys *sys, size_t n, const double *pos, const double *vel,
                    const double *mass);
+/* As glx_sys_upload, with the state drawn on the device by a generator of galaxy_ic.h
+ * (same values as glx_ic_generate on the host): no host arrays, no upload. The system
+ * keeps the g it was created with. The ic crate samples on the host only
+ * (ic/src/lib.rs:9-33); this is the device counterpart of upload(sample(...)). */
+int glx_sys_generate(glx_sys *sys, const struct glx_ic_spec *spec);
 /* step / step_many (gpu/src/gpu_resident.rs:1823,1848): k KDK steps of dt. */
 int glx_sys_step(glx_sys *sys, double dt, uint64_t k);
 /* As glx_sys_step, launched without the graph and with a HIP event between phases on
@@ -212,6 +225,10 @@ int glx_sys_count(glx_sys *sys, glx_stats *out);
 int glx_sys_snapshot(glx_sys *sys, double *pos, double *vel);
 /* snapshot_accel (gpu/src/gpu_resident.rs:1655): cached a (3n doubles). */
 int glx_sys_snapshot_accel(glx_sys *sys, double *acc);
+/* Masses (n doubles). The reference tracks them on the host (snapshot,
+ * gpu/src/gpu_resident.rs:1876); here they are read from the device, where
+ * glx_sys_generate writes them. */
+int glx_sys_snapshot_mass(glx_sys *sys, double *mass);
 /* time / len (gpu/src/gpu_resident.rs:1947,1960). */
 int glx_sys_time(glx_sys *sys, double *time);
 int glx_sys_len(glx_sys *sys, size_t *n);
@@ -261,6 +278,11 @@ int glx_pool_destroy(glx_pool *pool);
 /* Uploads the job; its a(x0) is computed before the next step, in one batched launch with
  * the other jobs admitted since. GLX_EBUDGET / GLX_ENOSLOT when it does not fit. */
 int glx_pool_admit(glx_pool *pool, const glx_job *job, uint64_t *id);
+/* As glx_pool_admit, with the job's state drawn on the device by a generator of
+ * galaxy_ic.h (no host arrays, no upload); the job's g is the generator's (spec->g, 1 for
+ * a collision). */
+int glx_pool_admit_ic(glx_pool *pool, const struct glx_ic_spec *spec, double softening,
+                      double theta, double dt, uint64_t steps, uint64_t *id);
 /* One KDK step of every active job; device_ms (may be NULL) = device time of the step. */
 int glx_pool_step(glx_pool *pool, double *device_ms);
 /* One step with direct launches and per-phase device ms (see glx_sys_step_timed). */

@@ -1,4 +1,4 @@
-"""ctypes front end of the CPU oracle (oracle/lbvh_oracle.c).
+"""ctypes front end of the CPU oracle (oracle/lbvh_oracle.c, oracle/ic_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the timed CPU
 baseline. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
@@ -76,6 +76,14 @@ def lib():
         L.orc_reference_morton63.argtypes = [S, P, P, P, P]
         L.orc_reference_sort63.argtypes = [S, P, P]
         L.orc_reference_karras63.argtypes = [S, P, P, P, P]
+        L.orc_ic_hernquist_df.argtypes = [D, D, D, D]
+        L.orc_ic_hernquist_df.restype = D
+        L.orc_ic_galaxy_body.argtypes = [S, C.c_uint64, D, P, P, P]
+        L.orc_ic_galaxy_body.restype = None
+        L.orc_ic_sphere.argtypes = [C.c_int, S, C.c_uint64, D, D, D, D, P, P, P]
+        L.orc_ic_sphere.restype = None
+        L.orc_ic_collision.argtypes = [S, C.c_uint64, D, D, D, P, P, P, P]
+        L.orc_ic_collision.restype = C.c_int
         assert L.orc_node_size() == NODE_DTYPE.itemsize
         _lib = L
     return _lib
@@ -284,3 +292,38 @@ def kdk_run(pos, vel, mass, g, softening, theta, dt, nsteps, solver="lbvh", nthr
     if rc:
         raise MemoryError(f"oracle build failed ({rc})")
     return pos, vel, acc
+
+
+# --- workload generators with one stream per particle (oracle/ic_oracle.c) ---
+
+def ic_hernquist_df(g, m, a, energy) -> float:
+    """ic/src/hernquist.rs:78-96"""
+    return float(lib().orc_ic_hernquist_df(g, m, a, energy))
+
+
+def ic_sphere(kind, n, seed, g=1.0, m=1.0, a=1.0, rmax=0.0):
+    """kind 0: Plummer (ic/src/plummer.rs:94-139, truncated at rmax > 0), kind 1:
+    Hernquist (ic/src/hernquist.rs:124-176); recentred -> (pos, vel, mass)."""
+    pos, vel, mass = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    lib().orc_ic_sphere(int(kind), n, seed & 0xFFFFFFFFFFFFFFFF, g, m, a, rmax, _p(pos),
+                        _p(vel), _p(mass))
+    return pos, vel, mass
+
+
+def ic_galaxy_body(n, seed, scale=1.0):
+    """One collision galaxy in its body frame before recentring (ic/src/disk.rs:333-371,
+    ic/src/hernquist.rs:139-176) -> (pos, vel, mass)."""
+    pos, vel, mass = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    lib().orc_ic_galaxy_body(n, seed & 0xFFFFFFFFFFFFFFFF, scale, _p(pos), _p(vel), _p(mass))
+    return pos, vel, mass
+
+
+def ic_collision(n, seed, mass_ratio=1.0, rp=8.0, r0=80.0):
+    """Two galaxies on a parabolic orbit (ic/src/encounter.rs:45-87, ic/src/orient.rs:63-71)
+    -> (pos, vel, mass, info {incl1, arg1, incl2, arg2})."""
+    pos, vel, mass, info = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(4)
+    rc = lib().orc_ic_collision(n, seed & 0xFFFFFFFFFFFFFFFF, mass_ratio, rp, r0, _p(pos),
+                                _p(vel), _p(mass), _p(info))
+    if rc:
+        raise ValueError("invalid collision parameters")
+    return pos, vel, mass, info
