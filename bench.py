@@ -54,6 +54,7 @@ FLOPS_L2P = 60
 # reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
 # written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
 WALK_TRAFFIC_BYTES = 51.06e9 + 0.557e9
+WALK_TRAFFIC_CONFIG = (1, 0.5, 0.0)  # (leaf size, far_theta, mono_ratio) it was measured on
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
@@ -468,6 +469,8 @@ def main():
                     help="walk leaf size of the ensemble (0: autotune on 64 of its systems)")
     ap.add_argument("--far-theta", type=float, default=-1.0,
                     help="far-field ratio of the group walk (0: off; negative: autotune)")
+    ap.add_argument("--mono-ratio", type=float, default=-1.0,
+                    help="monopole band of the group walk (0: off; negative: autotune)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
     ap.add_argument("--serve-device-ic", action="store_true",
                     help="serving jobs drawn on the device at admission (glx_pool_admit_ic)")
@@ -530,20 +533,22 @@ def main():
     log(f"ensemble: {len(systems)} systems, {int(sizes.sum())} particles, "
         f"ICs {time.perf_counter() - t0:.1f}s")
     tuning = None
-    far = max(args.far_theta, 0.0)
-    if args.leaf_size and args.far_theta >= 0:
-        leaf = args.leaf_size
-    else:
+    leaf, far, mono = args.leaf_size, max(args.far_theta, 0.0), max(args.mono_ratio, 0.0)
+    if not (args.leaf_size and args.far_theta >= 0 and args.mono_ratio >= 0):
         from galaxy_amd.tune import tune_leaf_size
 
         pick = np.random.default_rng(args.seed + 7).choice(len(systems), 64, replace=False)
+        t0 = time.perf_counter()
         tuning = tune_leaf_size([systems[k] for k in sorted(pick)], softening=eps, theta=theta,
                                 dt=dt, leaf_sizes=(args.leaf_size,) if args.leaf_size else None,
-                                far_thetas=(args.far_theta,) if args.far_theta >= 0 else None)
-        leaf, far = tuning["leaf_size"], tuning["far_theta"]
-        log(f"tuned leaf size {leaf}, far_theta {far}: {tuning['table']}")
+                                far_thetas=(args.far_theta,) if args.far_theta >= 0 else None,
+                                mono_ratios=(args.mono_ratio,) if args.mono_ratio >= 0 else None)
+        leaf, far, mono = tuning["leaf_size"], tuning["far_theta"], tuning["mono_ratio"]
+        log(f"tuned in {time.perf_counter() - t0:.1f}s: leaf size {leaf}, far_theta {far}, "
+            f"mono_ratio {mono}: {tuning['table']}")
     fast_mode["leaf_size"] = leaf
     fast_mode["far_theta"] = far
+    fast_mode["mono_ratio"] = mono
     b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, fast_mode, 8)
     log(f"batched: {b['value']:.4g} pf/s, {b['ms_per_step']:.2f} ms/step, err {b['error']}")
     out = {
@@ -564,7 +569,7 @@ def main():
                    "systems": len(systems), "particles": int(sizes.sum()),
                    "n_range": [1024, 65536], "theta": theta, "eps": eps, "dt": dt,
                    "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
-                   "leaf_size": leaf, "far_theta": far},
+                   "leaf_size": leaf, "far_theta": far, "mono_ratio": mono},
         "p50_step_ms": b["device_ms"]["p50"],
         "p99_step_ms": b["device_ms"]["p99"],
         "force_error_vs_fp64_direct": b["error"],
@@ -576,7 +581,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
             "frac_vs_non_packed_peak": b["walk"]["tflops"] / FP32_NONPACKED_TFLOPS,
-            "traffic": WALK_TRAFFIC_BYTES if (leaf, far) == (1, 0.5) else None,
+            "traffic": WALK_TRAFFIC_BYTES if (leaf, far, mono) == WALK_TRAFFIC_CONFIG else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
                               " leaf size 1, far-field ratio 0.5",
             "flops_per_launch": b["walk"]["flops"],

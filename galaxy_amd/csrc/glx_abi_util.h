@@ -58,6 +58,10 @@ inline void validate_params(const glx_params* p) {
     fail(GLX_EINVAL, "far_theta must be in [0, 1)");
   if (p->far_theta > 0.0 && (p->walk != GLX_WALK_GROUP || p->multipole != 2))
     fail(GLX_EINVAL, "far_theta needs the group walk with quadrupoles");
+  if (!isfinite(p->mono_ratio) || p->mono_ratio < 0.0 || p->mono_ratio >= 1.0)
+    fail(GLX_EINVAL, "mono_ratio must be in [0, 1)");
+  if (p->mono_ratio > 0.0 && p->multipole != 2)
+    fail(GLX_EINVAL, "mono_ratio needs quadrupoles");
 }
 
 inline Mode mode_from(const glx_params& p) {
@@ -82,6 +86,7 @@ inline void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
   d.precision = p.precision;
   d.leaf_size = p.leaf_size ? p.leaf_size : 1u;
   d.far2 = (float)(p.far_theta * p.far_theta);
+  d.mono_inv = p.mono_ratio > 0.0 ? (float)(1.0 / (p.theta * p.mono_ratio)) : 0.0f;
 }
 
 inline hipStream_t make_stream() {

@@ -91,6 +91,7 @@ struct SegDesc {
   uint32_t leaf_size;  // group walk: nodes of at most this many bodies are summed body by body
   float g, eps2, theta, half_dt;
   float far2;  // squared far-field ratio: (node radius + group radius)^2 <= far2 * R^2
+  float mono_inv;  // 1 / (theta mono_ratio): accepted nodes this far are monopoles (0: off)
   double dt;
 };
 

@@ -1158,10 +1158,18 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       const uint32_t g0 = tl.start + l0;
       const uint32_t gl = (uint32_t)__shfl((int)ge, (int)l0);
       const uint32_t gcount = min(min(gl - g0 + 1u, (uint32_t)kGroup), n - g0);
-      const bool valid = lane < gcount;
-      const uint32_t k = g0 + (valid ? lane : 0u);
+      // A group of at most 32 bodies gives each body rep lanes (2, 3 or 4): the lists are
+      // drained with stride rep, and the partial sums are folded into the body's first
+      // lane (its owner) before the store.
+      const uint32_t rep = gcount > 32u ? 1u : (gcount > 21u ? 2u : (gcount > 16u ? 3u : 4u));
+      const uint32_t sub = (uint32_t)(lane >= gcount) + (uint32_t)(lane >= 2u * gcount) +
+                           (uint32_t)(lane >= 3u * gcount);
+      const bool owner = lane < gcount;
+      const bool valid = lane < rep * gcount;
+      const uint32_t k = g0 + (valid ? lane - sub * gcount : 0u);
+      const uint32_t e0 = valid ? sub : 0xFFFFu;  // this lane's first list entry
       if (s.active) {  // masked walk: a group without an active body is skipped
-        const bool on = valid && s.active[min(s.vals[0][k], n - 1)] != 0;
+        const bool on = owner && s.active[min(s.vals[0][k], n - 1)] != 0;
         if (__ballot(on) == 0ull) continue;
       }
       const float4 me = s.leaf[k];
@@ -1218,7 +1226,6 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool near = node_live & !accept;
         const bool bkt = near & (ncount <= bucket);
         const bool open = near & !bkt;
-        const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept);
         // act 5: accepted and well separated from the whole group -> local expansion
         bool far = false;
         if (FAR) {
@@ -1228,6 +1235,17 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
           far = accept & (rn * rn <= s.far2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
         }
+        // an accepted node that also passes the opening test at theta * mono_ratio is a
+        // monopole (act 1); s = largest AABB side, delta = |com - AABB centre|
+        bool mono = false;
+        if (QUAD) {
+          const float side = fmaxf(fmaxf(Cc.x - B.x, Cc.y - B.y), Cc.z - B.z);
+          const float ox = A.x - 0.5f * (B.x + Cc.x), oy = A.y - 0.5f * (B.y + Cc.y),
+                      oz = A.z - 0.5f * (B.z + Cc.z);
+          const float rm = fmaf(side, s.mono_inv, sqrtf(fmaf(ox, ox, fmaf(oy, oy, oz * oz))));
+          mono = accept & (s.mono_inv > 0.0f) & (dmin2 >= rm * rm);
+        }
+        const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept) | (mono & !far);
         const int act = point ? 1 : (accept ? (far ? 5 : 2) : (open ? 3 : (bkt ? 4 : 0)));
         const float4 P = is_leaf ? LP : A;
         const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
@@ -1265,19 +1283,22 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         __syncthreads();
         if (np != 0u) {
 #pragma unroll 2
-          for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
-          if (COUNT) n_int += np;
+          for (uint32_t e = e0; e < np; e += rep) point_term<KAHAN>(a, me, pl[e], eps2);
+          if (COUNT && e0 < np) n_int += (np - e0 + rep - 1u) / rep;
           np = 0;
         }
-        if (nr != 0u) {
-          const uint32_t c = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
-          if (COUNT) n_int += c;
+        if (nr != 0u) {  // bucket bodies come through scalar loads: owners only
+          if (owner) {
+            const uint32_t c = bucket_terms<KAHAN>(a, me, s, rl, nr, eps2);
+            if (COUNT) n_int += c;
+          }
           nr = 0;
         }
         if (QUAD && nq != 0u) {
 #pragma unroll 2
-          for (uint32_t e = 0; e < nq; ++e) quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
-          if (COUNT) n_quad += nq;
+          for (uint32_t e = e0; e < nq; e += rep)
+            quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
+          if (COUNT && e0 < nq) n_quad += (nq - e0 + rep - 1u) / rep;
           nq = 0;
         }
         if (FAR && nf >= 64u) {  // one far node per lane
@@ -1291,9 +1312,18 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         n_int = 0;
         n_quad = 0;
-        if (valid) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
+        if (owner) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
         if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
       }  // otherwise the lists were drained in the last round
+      if (rep > 1u) {  // fold each body's lanes into its owner (lanes lane + j gcount < 64)
+        const float px = a.x - (KAHAN ? a.cx : 0.f), py = a.y - (KAHAN ? a.cy : 0.f),
+                    pz = a.z - (KAHAN ? a.cz : 0.f);
+        for (uint32_t j = 1; j < rep; ++j) {
+          const int src = (int)((lane + j * gcount) & 63u);
+          const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+          if (owner) acc_add<KAHAN>(a, qx, qy, qz);
+        }
+      }
       if (FAR && !overflow) {
         far_drain(loc, s, fl, nf, lane, cx, cy, cz, eps2);
         if (COUNT && lane == 0) n_far += nf;
@@ -1303,7 +1333,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         far_l2p(loc, me.x - cx, me.y - cy, me.z - cz, fx, fy, fz);
         acc_add<KAHAN>(a, fx, fy, fz);
       }
-      if (valid) store_acc<KAHAN>(s, k, a, mode);
+      if (owner) store_acc<KAHAN>(s, k, a, mode);
       if (COUNT) {
         // n_int is per target (uniform over valid lanes unless the group fell back)
         t_int += wave_sum_u64(valid ? n_int : 0ull);

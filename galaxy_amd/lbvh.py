@@ -44,6 +44,7 @@ class Params(C.Structure):
         ("morton_bits", C.c_uint32),
         ("walk", C.c_uint32),
         ("far_theta", C.c_double),
+        ("mono_ratio", C.c_double),
     ]
 
 
@@ -160,10 +161,11 @@ def device_info():
 
 
 def _params(g, softening, theta, precision=PREC_F32, count=False, morton_bits=30,
-            walk=WALK_REFERENCE, multipole=0, leaf_size=1, far_theta=0.0) -> Params:
+            walk=WALK_REFERENCE, multipole=0, leaf_size=1, far_theta=0.0,
+            mono_ratio=0.0) -> Params:
     return Params(float(g), float(softening), float(theta), int(leaf_size), int(multipole),
                   int(precision), FLAG_COUNT if count else 0, int(morton_bits), int(walk),
-                  float(far_theta))
+                  float(far_theta), float(mono_ratio))
 
 
 @dataclass
@@ -185,11 +187,12 @@ class Lbvh:
 
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
                  count: bool = False, morton_bits: int = 30, walk: int = WALK_REFERENCE,
-                 multipole: int = 0, leaf_size: int = 1, far_theta: float = 0.0):
+                 multipole: int = 0, leaf_size: int = 1, far_theta: float = 0.0,
+                 mono_ratio: float = 0.0):
         self.g, self.softening, self.theta = float(g), float(softening), float(theta)
         h = C.c_void_p()
         p = _params(g, softening, theta, precision, count, morton_bits, walk, multipole,
-                    leaf_size, far_theta)
+                    leaf_size, far_theta, mono_ratio)
         _check(lib().glx_solver_create(C.byref(p), C.byref(h)))
         self._h = h
 
@@ -363,10 +366,10 @@ class GpuResidentLeapfrog:
 
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
                  morton_bits: int = 30, walk: int = WALK_REFERENCE, multipole: int = 0,
-                 leaf_size: int = 1, far_theta: float = 0.0):
+                 leaf_size: int = 1, far_theta: float = 0.0, mono_ratio: float = 0.0):
         h = C.c_void_p()
         p = _params(g, softening, theta, precision, False, morton_bits, walk, multipole,
-                    leaf_size, far_theta)
+                    leaf_size, far_theta, mono_ratio)
         _check(lib().glx_sys_create(C.byref(p), C.byref(h)))
         self._h = h
         self._mass = np.zeros(0)

@@ -61,6 +61,11 @@ typedef struct glx_params {
                            far_theta times its distance from the group centre enters one
                            second-order local expansion per group instead of a term per
                            body */
+  double mono_ratio;    /* 0 = off; in (0, 1) with multipole 2: an accepted node that also
+                           passes the opening test at theta * mono_ratio (distance >=
+                           s / (theta mono_ratio) + delta from every point of the group) is
+                           summed as a monopole alone; nearer accepted nodes keep the
+                           quadrupole */
 } glx_params;
 
 /* FP64 conservation diagnostics (core/src/diagnostics.rs:8-68). */

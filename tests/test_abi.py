@@ -78,6 +78,9 @@ def test_invalid_params_rejected_before_device():
         L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 0, 0.4),  # ... and the group walk
         L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, 1.0),  # far_theta >= 1
         L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, float("nan")),
+        L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, 0.0, 1.0),  # mono_ratio >= 1
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 63, 1, 0.0, 0.5),  # ... needs quadrupoles
+        L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, 0.0, -0.1),
     ):
         assert lib.glx_solver_create(C.byref(bad), C.byref(h)) == L.GLX_EINVAL
         assert lib.glx_last_error()

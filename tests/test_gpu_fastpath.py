@@ -269,6 +269,47 @@ def test_far_field_error_and_work():
         assert np.array_equal(a, s.accelerations(pos, mass))  # deterministic
 
 
+def test_monopole_band_limits():
+    """mono_ratio -> 0 keeps every quadrupole (bit-equal to the band off); mono_ratio -> 1
+    turns (nearly) every accepted node into a monopole, i.e. the group walk without
+    quadrupoles; theta -> 0 accepts nothing, so the direct sum."""
+    pos, _, mass = I.plummer_numpy(8192, seed=4)
+    off = glx.Lbvh(1.0, 0.01, 0.5, **FAST).accelerations(pos, mass)
+    tiny = glx.Lbvh(1.0, 0.01, 0.5, **dict(FAST, mono_ratio=1e-6)).accelerations(pos, mass)
+    assert np.array_equal(off, tiny)
+    mono_walk = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=0)
+    m0 = glx.Lbvh(1.0, 0.01, 0.5, **mono_walk).accelerations(pos, mass)
+    m1 = glx.Lbvh(1.0, 0.01, 0.5, **dict(FAST, mono_ratio=0.999999)).accelerations(pos, mass)
+    d = np.linalg.norm(m1 - m0, axis=1) / np.linalg.norm(m0, axis=1)
+    say(f"mono_ratio ~1 vs monopole group walk: median {np.median(d):.3g} max {d.max():.3g}")
+    assert np.median(d) < 1e-5
+    s = glx.Lbvh(G, EPS, 1e-6, **dict(FAST, mono_ratio=0.9))
+    for seed in range(3):
+        p, _, m = I.cluster(seed, 300)
+        assert I.rms_rel_err(s.accelerations(p, m), O.direct_sum(p, m, G, EPS)) < 2e-5
+
+
+def test_monopole_band_error_and_work():
+    """Plummer N=16384, theta=0.5 against the FP64 direct sum: the band converts accepted
+    quadrupoles into monopoles and the error grows with mono_ratio, staying below the
+    monopole-only group walk's."""
+    pos, _, mass = I.plummer_numpy(16384, seed=9)
+    eps = 0.01
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, eps, np.arange(16384, dtype=np.uint32))
+    mono_walk = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=0)
+    e_mono = _errs(glx.Lbvh(1.0, eps, 0.5, **mono_walk).accelerations(pos, mass), d64)
+    prev, prev_q = None, None
+    for mr, med_max in ((0.0, 2e-4), (0.7, 5e-4), (0.8, 7e-4), (0.9, 9e-4)):
+        s = glx.Lbvh(1.0, eps, 0.5, count=True, **dict(FAST, mono_ratio=mr))
+        e = _errs(s.accelerations(pos, mass), d64)
+        st = s.stats()
+        say(f"mono_ratio {mr}: err {e} stats {st} (monopole walk {e_mono})")
+        assert e[0] <= med_max and e[0] < e_mono[0]
+        if prev is not None:
+            assert e[0] >= prev * 0.98 and st["quad_terms"] < prev_q
+        prev, prev_q = e[0], st["quad_terms"]
+
+
 def test_far_field_collision_error():
     pos, _, mass, _ = ic.collision(1 << 18, 12)
     rng = np.random.default_rng(2)
