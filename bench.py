@@ -53,8 +53,8 @@ FLOPS_L2P = 60
 # by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
 # reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
 # written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 51.06e9 + 0.557e9
-WALK_TRAFFIC_CONFIG = (1, 0.5, 0.0)  # (leaf size, far_theta, mono_ratio) it was measured on
+WALK_TRAFFIC_BYTES = 43.68e9 + 0.558e9
+WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8)  # (leaf size, far_theta, mono_ratio) it was measured on
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
@@ -519,6 +519,7 @@ def main():
         return
     if args.profile_collision:
         fast_mode["far_theta"] = max(args.far_theta, 0.0)
+        fast_mode["mono_ratio"] = max(args.mono_ratio, 0.0)
         n = args.collision_n
         pos, vel, mass, _ = ic.collision(n, args.seed)
         tg = np.arange(min(n, 256), dtype=np.uint32)
@@ -583,7 +584,7 @@ def main():
             "frac_vs_non_packed_peak": b["walk"]["tflops"] / FP32_NONPACKED_TFLOPS,
             "traffic": WALK_TRAFFIC_BYTES if (leaf, far, mono) == WALK_TRAFFIC_CONFIG else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
-                              " leaf size 1, far-field ratio 0.5",
+                              " leaf size 1, far-field ratio 0.55, monopole band 0.8",
             "flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
         },
