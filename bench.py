@@ -55,6 +55,7 @@ FLOPS_L2P = 60
 # written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
 WALK_TRAFFIC_BYTES = 43.68e9 + 0.558e9
 WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8)  # (leaf size, far_theta, mono_ratio) it was measured on
+WALK_VALU_ISSUE = 0.92  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
@@ -587,6 +588,10 @@ def main():
                               " leaf size 1, far-field ratio 0.55, monopole band 0.8",
             "flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
+            # VALU wave-instructions issued per launch / what 256 CUs issue in its duration
+            # (rocprofv3 SQ_INSTS_VALU, profiles/sq_counters_walk_ensemble512.md): the
+            # walk is issue-bound although its counted flops are a small fraction of peak
+            "valu_issue_frac": WALK_VALU_ISSUE if (leaf, far, mono) == WALK_TRAFFIC_CONFIG else None,
         },
         "batched": strip(b),
         "leaf_size_tuning": tuning,
