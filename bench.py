@@ -379,10 +379,15 @@ def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nm
     admitted = finished = rejected_wait = 0
     queue_delay = []
     t_start = time.perf_counter()
+    next_report = 30.0
     while True:
         now = time.perf_counter() - t_start
         if now >= seconds:
             break
+        if now >= next_report:  # progress of a long stream
+            log(f"serving {now:.0f}s: admitted {admitted}, finished {finished}, active "
+                f"{len(active)}, steps {len(lat)}")
+            next_report += 30.0
         while not ready.empty():
             waiting.append(ready.get())
         while pending and arrivals[pending[0]] <= now:
