@@ -490,10 +490,26 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 //    as the right child of its split position, from which the skip pointers follow.
 // ----------------------------------------------------------------------------------
 
+// The sorted keys of a tile and kKarrasPad on either side, staged in LDS: the searches
+// of most nodes stay inside this window and only the long ranges of nodes near the root
+// read global memory.
+constexpr int kKarrasPad = 512;
+
 template <class K>
-__device__ __forceinline__ int kdelta(const K* __restrict__ codes, int n, int a, int b) {
+struct KeyWindow {
+  const K* codes;
+  const K* win;
+  int w0, wlen;
+  __device__ __forceinline__ K at(int g) const {
+    const int q = g - w0;
+    return (q >= 0 && q < wlen) ? win[q] : codes[g];
+  }
+};
+
+template <class K>
+__device__ __forceinline__ int kdelta(const KeyWindow<K>& kw, int n, int a, int b) {
   if (b < 0 || b >= n) return -1;
-  const K ca = codes[a], cb = codes[b];
+  const K ca = kw.at(a), cb = kw.at(b);
   if (ca == cb) return KeyTraits<K>::tie_base + __clz((int)((uint32_t)a ^ (uint32_t)b));
   return KeyTraits<K>::clz(ca ^ cb);
 }
@@ -501,27 +517,34 @@ __device__ __forceinline__ int kdelta(const K* __restrict__ codes, int n, int a,
 template <class K>
 __global__ void __launch_bounds__(kBlock)
 k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
+  __shared__ K win[kSortTile + 2 * kKarrasPad];
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const int n = (int)s.n;
     const K* codes = static_cast<const K*>(s.keys[0]);
+    KeyWindow<K> kw{codes, win, (int)tl.start - kKarrasPad, (int)tl.count + 2 * kKarrasPad};
+    for (int q = threadIdx.x; q < kw.wlen; q += kBlock) {
+      const int g = kw.w0 + q;
+      if (g >= 0 && g < n) win[q] = codes[g];
+    }
+    __syncthreads();
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const int i = (int)(tl.start + e);
       if (i >= n - 1) continue;
-      const int dir = kdelta(codes, n, i, i + 1) > kdelta(codes, n, i, i - 1) ? 1 : -1;
-      const int dmin = kdelta(codes, n, i, i - dir);
+      const int dir = kdelta(kw, n, i, i + 1) > kdelta(kw, n, i, i - 1) ? 1 : -1;
+      const int dmin = kdelta(kw, n, i, i - dir);
       int lmax = 2;
-      while (kdelta(codes, n, i, i + lmax * dir) > dmin) lmax *= 2;
+      while (kdelta(kw, n, i, i + lmax * dir) > dmin) lmax *= 2;
       int l = 0;
       for (int tt = lmax / 2; tt > 0; tt /= 2)
-        if (kdelta(codes, n, i, i + (l + tt) * dir) > dmin) l += tt;
+        if (kdelta(kw, n, i, i + (l + tt) * dir) > dmin) l += tt;
       const int j = i + l * dir;
-      const int dnode = kdelta(codes, n, i, j);
+      const int dnode = kdelta(kw, n, i, j);
       int sp = 0, tt = l;
       for (;;) {
         tt = (tt + 1) / 2;
-        if (kdelta(codes, n, i, i + (sp + tt) * dir) > dnode) sp += tt;
+        if (kdelta(kw, n, i, i + (sp + tt) * dir) > dnode) sp += tt;
         if (tt <= 1) break;
       }
       const int gamma = i + sp * dir + (dir < 0 ? dir : 0);
@@ -546,6 +569,7 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
         s.gend[0] = (uint32_t)(n - 1);
       }
     }
+    __syncthreads();  // the window is refilled for the next tile
   }
 }
 
