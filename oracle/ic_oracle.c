@@ -17,6 +17,20 @@
  *   orientation Rz(w) Rx(i)     ic/src/orient.rs:63-71
  * and the particle streams are keyed as the library keys them: (seed, component, index)
  * through two rounds of the SplitMix64 finaliser.
+ *
+ * Pinning (tests/test_ic_streams.py, what the mathematics fixes, after the reference's
+ * own sampler tests):
+ *   Plummer      enclosed-mass fractions at 5 radii (4-sigma binomial bands), half-mass
+ *                radius (3%), virial ratio against W = 3 pi/32 (5%), speed distribution
+ *                q^2 (1 - q^2)^3.5 (KS), truncation radius.
+ *   Hernquist    DF integrates to the closed-form density (3e-3), enclosed-mass
+ *                fractions of the 300a-truncated profile, half-mass radius, virial ratio
+ *                against W = 1/6.
+ *   galaxy       disk surface density and sech^2 height fractions, v_phi about the
+ *                closed-form circular speed with 10% dispersion, bulge and halo profiles,
+ *                component masses.
+ *   collision    parabolic relative orbit (zero energy, h^2 = 2 mu r_p, approaching),
+ *                zero COM and momentum, disk spin along (sin i sin w, -sin i cos w, cos i).
  */
 #include <math.h>
 #include <stddef.h>
