@@ -596,7 +596,8 @@ def main():
             # VALU wave-instructions issued per launch / what 256 CUs issue in its duration
             # (rocprofv3 SQ_INSTS_VALU, profiles/sq_counters_walk_ensemble512.md): the
             # walk is issue-bound although its counted flops are a small fraction of peak
-            "valu_issue_frac": WALK_VALU_ISSUE if (leaf, far, mono) == WALK_TRAFFIC_CONFIG else None,
+            "valu_issue_frac": (WALK_VALU_ISSUE if (leaf, far, mono) == WALK_TRAFFIC_CONFIG
+                                else None),
         },
         "batched": strip(b),
         "leaf_size_tuning": tuning,

@@ -288,7 +288,8 @@ k_iota(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) s.vals[0][tl.start + e] = tl.start + e;
+    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock)
+      s.vals[0][tl.start + e] = tl.start + e;
   }
 }
 
@@ -325,7 +326,8 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanChunks = kScanThreads / kRadixBins;
 
 __global__ void __launch_bounds__(kScanThreads)
-k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ tile_hist,
+k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg,
+             const uint32_t* __restrict__ tile_hist,
              uint32_t* __restrict__ tile_off) {
   __shared__ uint32_t part[kScanChunks][kRadixBins];
   __shared__ uint32_t base[kRadixBins];
@@ -1048,7 +1050,8 @@ __device__ __forceinline__ void quad_term(Acc3& a, const float4& me, const float
 // folded into a second-order local expansion of the softened acceleration about the
 // centre, a(c + y) = A0 + A1 y + A2 y y / 2 (R^2 = |d|^2 + eps^2):
 //   monopole   A0 = m d / R^3,  A1_ij = m (3 d_i d_j / R^5 - delta_ij / R^3),
-//              A2_ijk = m (15 d_i d_j d_k / R^7 - 3 (delta_ij d_k + delta_ik d_j + delta_jk d_i) / R^5)
+//              A2_ijk = m (15 d_i d_j d_k / R^7
+//                          - 3 (delta_ij d_k + delta_ik d_j + delta_jk d_i) / R^5)
 //   quadrupole A0 = -q / R^5 + 2.5 s d / R^7 (q = Q d, s = d.q),
 //              A1_ij = Q_ij / R^5 - 5 (q_i d_j + q_j d_i) / R^7 - 2.5 s delta_ij / R^7
 //                      + 17.5 s d_i d_j / R^9
@@ -1657,7 +1660,8 @@ static void walk_group_q(hipStream_t st, bool quad, bool far, const SegDesc* seg
   else if (quad)
     hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true, false>), g, b, 0, st, segs, gt, ngt, mode);
   else
-    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false, false>), g, b, 0, st, segs, gt, ngt, mode);
+    hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, false, false>), g, b, 0, st, segs, gt, ngt,
+                       mode);
 }
 
 hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* wt,

@@ -171,7 +171,9 @@ extern "C" int glx_pool_admit(glx_pool* p, const glx_job* job, uint64_t* id) {
     if (job->n == 0) fail(GLX_EINVAL, "a job needs at least one particle");
     if (!job->pos || !job->vel || !job->mass) fail(GLX_EINVAL, "NULL array");
     admit_job(p, job->n, job->g, job->softening, job->theta, job->dt, job->steps, id,
-              [&](const SegDesc& d) { upload_state(d, job->n, job->pos, job->vel, job->mass, p->st); });
+              [&](const SegDesc& d) {
+                upload_state(d, job->n, job->pos, job->vel, job->mass, p->st);
+              });
   });
 }
 

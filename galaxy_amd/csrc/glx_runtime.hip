@@ -226,7 +226,8 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   grow(wtiles_, cap_wt_, n_wt_ ? n_wt_ : 1);
   grow(gtiles_, cap_gt_, n_gt_ ? n_gt_ : 1);
   auto up = [&](void* dst, const void* src, size_t bytes) {
-    if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "upload table");
+    if (bytes)
+      check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "upload table");
   };
   up(segs_, host_.data(), nseg_ * sizeof(SegDesc));
   up(etiles_, et.data(), n_et_ * sizeof(Tile));

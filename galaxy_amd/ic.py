@@ -60,7 +60,8 @@ def _bind():
         L.glx_ic_plummer.argtypes = [S, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double,
                                      P, P, P]
         L.glx_ic_collision.restype = C.c_int
-        L.glx_ic_collision.argtypes = [S, C.c_uint64, C.c_double, C.c_double, C.c_double, P, P, P, P]
+        L.glx_ic_collision.argtypes = [S, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                       P, P, P, P]
         L.glx_ic_generate.restype = C.c_int
         L.glx_ic_generate.argtypes = [P, P, P, P, P]
         L.glx_ic_generate_device.restype = C.c_int
@@ -91,8 +92,8 @@ def collision(n: int, seed: int, mass_ratio: float = 1.0, rp: float = 8.0, r0: f
     vel = np.zeros((n, 3))
     mass = np.zeros(n)
     info = np.zeros(4)
-    rc = _bind().glx_ic_collision(n, seed & 0xFFFFFFFFFFFFFFFF, mass_ratio, rp, r0, _p(pos), _p(vel),
-                                  _p(mass), _p(info))
+    rc = _bind().glx_ic_collision(n, seed & 0xFFFFFFFFFFFFFFFF, mass_ratio, rp, r0, _p(pos),
+                                  _p(vel), _p(mass), _p(info))
     if rc:
         raise ValueError("invalid collision parameters")
     return pos, vel, mass, {"incl": (info[0], info[2]), "arg": (info[1], info[3])}
