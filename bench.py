@@ -123,7 +123,19 @@ def cpu_ensemble(systems, eps, theta, budget_s, seed=3):
                    f"seeded order ({n_done} particles, {t_done:.1f} s)"),
         "all_cores": {"value": n_mt / t_mt, "cores": cores,
                       "sample": f"first {max(1, len(used) // 2)} of the same systems"},
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_model() -> str:
+    """Host CPU model name (/proc/cpuinfo), for the baseline's record."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_collision(pos, mass, eps, theta, gpu_acc, budget_s, seed=7):
