@@ -31,8 +31,10 @@ constexpr int kGroup = 64;                             // targets per group walk
 struct __align__(16) Node {
   float4 a;  // centre of mass (origin-relative f32), mass
   float4 b;  // AABB min (origin-relative f32), squared acceptance radius (s/theta + delta)^2
-  float4 c;  // AABB max (origin-relative f32), left child (unified index, as bits)
-  uint4 d;   // next node after this subtree (unified or kEnd), right child, first leaf, last leaf
+  float4 c;  // AABB max (origin-relative f32), children: split | leaf flags (as bits)
+  uint4 d;   // next node after this subtree (unified or kEnd), right child, first leaf,
+             // last leaf; with quadrupoles the first two hold the group walk's squared
+             // monopole-band radius and the node's radius about its com (as bits)
 };
 
 // Build/walk configuration shared by all segments of a launch sequence.

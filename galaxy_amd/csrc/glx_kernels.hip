@@ -594,6 +594,20 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const 
   return (const __attribute__((address_space(1))) T*)p;
 }
 
+// Children of an internal node in one word: the split position g (left child g, right
+// child g + 1, each a leaf or an internal node) and two leaf flags (n <= 2^29).
+constexpr uint32_t kLeftLeaf = 1u << 30, kRightLeaf = 1u << 31, kSplitMask = kLeftLeaf - 1u;
+__device__ __forceinline__ uint32_t enc_children(uint32_t cl, uint32_t cr, uint32_t n) {
+  const uint32_t g = cl < n ? cl : cl - n;
+  return g | (cl < n ? kLeftLeaf : 0u) | (cr < n ? kRightLeaf : 0u);
+}
+__device__ __forceinline__ uint32_t left_child(uint32_t e, uint32_t n) {
+  return (e & kLeftLeaf) ? (e & kSplitMask) : n + (e & kSplitMask);
+}
+__device__ __forceinline__ uint32_t right_child(uint32_t e, uint32_t n) {
+  return (e & kRightLeaf) ? (e & kSplitMask) + 1u : n + (e & kSplitMask) + 1u;
+}
+
 struct AggV {
   float3 com, mn, mx;
   float mass;
@@ -664,8 +678,21 @@ __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const A
   Node nd;
   nd.a = make_float4(r.com.x, r.com.y, r.com.z, r.mass);
   nd.b = make_float4(r.mn.x, r.mn.y, r.mn.z, rcrit * rcrit);
-  nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(cl));
-  nd.d = make_uint4(nx, cr, lo, hi);
+  nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(enc_children(cl, cr, s.n)));
+  if (QUAD) {
+    // the group walk's classifier radii: squared monopole-band radius (s / (theta
+    // mono_ratio) + delta; FLT_MAX when the band is off) and the node's radius about its
+    // com (far-field test); the skip pointer is rebuilt from rs[hi] where needed
+    const float bx = fmaxf(r.com.x - r.mn.x, r.mx.x - r.com.x),
+                by = fmaxf(r.com.y - r.mn.y, r.mx.y - r.com.y),
+                bz = fmaxf(r.com.z - r.mn.z, r.mx.z - r.com.z);
+    const float rn = sqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
+    const float rm = fmaf(size, s.mono_inv, delta);
+    nd.d = make_uint4(__float_as_uint(s.mono_inv > 0.f ? rm * rm : 3.4e38f),
+                      __float_as_uint(rn), lo, hi);
+  } else {
+    nd.d = make_uint4(nx, cr, lo, hi);
+  }
   s.node[j] = nd;
   if (QUAD) {
     s.quad[2 * j] = make_float4(r.q[0], r.q[1], r.q[2], r.q[3]);
@@ -910,7 +937,7 @@ __device__ __forceinline__ void store_acc(const SegDesc& s, uint32_t k, const Ac
 template <bool KAHAN, bool COUNT>
 __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 me, Acc3& a,
                                           unsigned long long& n_int,
-                                          unsigned long long& n_vis) {
+                                          unsigned long long& n_vis, bool quad_records) {
   const uint32_t n = s.n;
   const float eps2 = s.eps2;
   const float4* __restrict__ leaf = s.leaf;
@@ -936,7 +963,8 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
     } else {
       const Node* nd = node + (u - n);
       const float4 A = nd->a;
-      const uint32_t nxt = nd->d.x;
+      // quadrupole builds keep classifier radii in d.xy: the skip pointer is rs[hi]
+      const uint32_t nxt = quad_records ? (nd->d.w == n - 1 ? kEnd : rs[nd->d.w]) : nd->d.x;
       if (!(A.w > 0.0f)) {
         u = nxt;
         continue;
@@ -954,7 +982,7 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
         if (COUNT) ++n_int;
         u = nxt;
       } else {
-        u = __float_as_uint(Cc.w);
+        u = left_child(__float_as_uint(Cc.w), n);
       }
     }
   }
@@ -973,7 +1001,7 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
     if (s.active && !s.active[min(s.vals[0][k], s.n - 1)]) continue;
     Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     unsigned long long n_int = 0, n_vis = 0;
-    lane_walk<KAHAN, COUNT>(s, k, s.leaf[k], a, n_int, n_vis);
+    lane_walk<KAHAN, COUNT>(s, k, s.leaf[k], a, n_int, n_vis, false);
     store_acc<KAHAN>(s, k, a, mode);
     if (COUNT) {
       atomicAdd(&s.counters[kCntInteractions], n_int);
@@ -1240,7 +1268,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const float4 LP = s.leaf[li];
         const Node& nd = s.node[n > 1 ? ni : 0u];
         const float4 A = nd.a, B = nd.b, Cc = nd.c;
-        const uint32_t crn = nd.d.y;
+        const uint4 D = nd.d;
         const float ex = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
         const float ey = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
         const float ez = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
@@ -1249,33 +1277,26 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
                            (Cc.z < gz0) | (B.z > gz1);
         const bool node_live = is_node & (A.w > 0.0f);
         const bool accept = node_live & apart & (dmin2 >= B.w);
-        const uint32_t nlo = nd.d.z, ncount = nd.d.w - nd.d.z + 1u;
+        const uint32_t nlo = D.z, ncount = D.w - D.z + 1u;
         const bool near = node_live & !accept;
         const bool bkt = near & (ncount <= bucket);
         const bool open = near & !bkt;
         // act 5: accepted and well separated from the whole group -> local expansion
+        // (node radius about its com from the record)
         bool far = false;
         if (FAR) {
-          const float bx = fmaxf(A.x - B.x, Cc.x - A.x), by = fmaxf(A.y - B.y, Cc.y - A.y),
-                      bz = fmaxf(A.z - B.z, Cc.z - A.z);
-          const float rn = sqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz))) + rg;
+          const float rn = __uint_as_float(D.y) + rg;
           const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
           far = accept & (rn * rn <= s.far2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
         }
         // an accepted node that also passes the opening test at theta * mono_ratio is a
-        // monopole (act 1); s = largest AABB side, delta = |com - AABB centre|
-        bool mono = false;
-        if (QUAD) {
-          const float side = fmaxf(fmaxf(Cc.x - B.x, Cc.y - B.y), Cc.z - B.z);
-          const float ox = A.x - 0.5f * (B.x + Cc.x), oy = A.y - 0.5f * (B.y + Cc.y),
-                      oz = A.z - 0.5f * (B.z + Cc.z);
-          const float rm = fmaf(side, s.mono_inv, sqrtf(fmaf(ox, ox, fmaf(oy, oy, oz * oz))));
-          mono = accept & (s.mono_inv > 0.0f) & (dmin2 >= rm * rm);
-        }
+        // monopole (act 1): the squared band radius is in the record
+        const bool mono = QUAD && accept && dmin2 >= __uint_as_float(D.x);
         const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept) | (mono & !far);
         const int act = point ? 1 : (accept ? (far ? 5 : 2) : (open ? 3 : (bkt ? 4 : 0)));
         const float4 P = is_leaf ? LP : A;
-        const uint32_t cl = __float_as_uint(Cc.w), cr = crn;
+        const uint32_t cl = left_child(__float_as_uint(Cc.w), n),
+                       cr = right_child(__float_as_uint(Cc.w), n);
         const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
                                  m3 = __ballot(act == 3), m4 = __ballot(act == 4);
         const uint32_t c3 = (uint32_t)__popcll(m3);
@@ -1339,7 +1360,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         n_int = 0;
         n_quad = 0;
-        if (owner) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis);
+        if (owner) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis, QUAD);
         if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
       }  // otherwise the lists were drained in the last round
       if (rep > 1u) {  // fold each body's lanes into its owner (lanes lane + j gcount < 64)
