@@ -608,6 +608,13 @@ __device__ __forceinline__ uint32_t right_child(uint32_t e, uint32_t n) {
   return (e & kRightLeaf) ? (e & kSplitMask) + 1u : n + (e & kSplitMask) + 1u;
 }
 
+// 16-byte record i of a float4 array as one global load (a flat load would also hold up
+// the LDS waits that follow it).
+__device__ __forceinline__ float4 gload4(const float4* p, size_t i) {
+  const auto* q = gbl(reinterpret_cast<const float*>(p)) + 4 * i;
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+
 struct AggV {
   float3 com, mn, mx;
   float mass;
@@ -1155,8 +1162,9 @@ __device__ __forceinline__ void far_drain(float* L, const SegDesc& s, const uint
                                           float cz, float eps2) {
   for (uint32_t e = lane; e < nf; e += kGroup) {
     const uint32_t j = fl[e];
-    const float4 A = s.node[j].a;
-    far_m2l(L, A.x - cx, A.y - cy, A.z - cz, A.w, s.quad[2 * j], s.quad[2 * j + 1], eps2);
+    const float4 A = gload4(reinterpret_cast<const float4*>(s.node), 4 * (size_t)j);
+    far_m2l(L, A.x - cx, A.y - cy, A.z - cz, A.w, gload4(s.quad, 2 * (size_t)j),
+            gload4(s.quad, 2 * (size_t)j + 1), eps2);
   }
 }
 
@@ -1201,9 +1209,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const uint32_t n = s.n;
     const float eps2 = s.eps2;
     const uint32_t bucket = s.leaf_size;
+    const bool loose_theta = s.theta > 0.55f;  // the disjoint-box test is needed
     // the walk groups (subtrees of at most kGroup bodies, marked by the tree build)
     // that start in this chunk of sorted leaves, one after the other
-    const uint32_t ge = lane < tl.count ? s.gend[tl.start + lane] : kEnd;
+    const uint32_t ge = lane < tl.count ? gbl(s.gend)[tl.start + lane] : kEnd;
     unsigned long long starts = __ballot(ge != kEnd);
     unsigned long long t_int = 0, t_vis = 0, t_quad = 0, t_far = 0;
     uint32_t t_peak = 0;
@@ -1227,7 +1236,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool on = owner && s.active[min(s.vals[0][k], n - 1)] != 0;
         if (__ballot(on) == 0ull) continue;
       }
-      const float4 me = s.leaf[k];
+      const float4 me = gload4(s.leaf, k);
       const float gx0 = wave_minf(me.x), gy0 = wave_minf(me.y), gz0 = wave_minf(me.z);
       const float gx1 = wave_maxf(me.x), gy1 = wave_maxf(me.y), gz1 = wave_maxf(me.z);
       // far field: group centre and radius; this lane's share of the local expansion
@@ -1265,16 +1274,23 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool is_node = live && u >= n;
         const uint32_t li = is_leaf ? u : 0u;
         const uint32_t ni = is_node ? u - n : 0u;
-        const float4 LP = s.leaf[li];
-        const Node& nd = s.node[n > 1 ? ni : 0u];
-        const float4 A = nd.a, B = nd.b, Cc = nd.c;
-        const uint4 D = nd.d;
+        const float4 LP = gload4(s.leaf, li);
+        const size_t nr4 = 4 * (size_t)(n > 1 ? ni : 0u);  // node record as four float4
+        const float4* nrec = reinterpret_cast<const float4*>(s.node);
+        const float4 A = gload4(nrec, nr4), B = gload4(nrec, nr4 + 1), Cc = gload4(nrec, nr4 + 2);
+        const float4 Df = gload4(nrec, nr4 + 3);
+        const uint4 D = make_uint4(__float_as_uint(Df.x), __float_as_uint(Df.y),
+                                   __float_as_uint(Df.z), __float_as_uint(Df.w));
         const float ex = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
         const float ey = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
         const float ez = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
         const float dmin2 = ex * ex + ey * ey + ez * ez;
-        const bool apart = (Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) | (B.y > gy1) |
-                           (Cc.z < gz0) | (B.z > gz1);
+        // The boxes must be disjoint. Every point of the node's box lies within sqrt(3) s of
+        // its com, and an accepted com is at least s / theta + delta from the group box, so
+        // for theta <= 0.55 (1 / theta > sqrt(3) with a margin) acceptance implies it.
+        const bool apart = loose_theta ? ((Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) |
+                                          (B.y > gy1) | (Cc.z < gz0) | (B.z > gz1))
+                                       : true;
         const bool node_live = is_node & (A.w > 0.0f);
         const bool accept = node_live & apart & (dmin2 >= B.w);
         const uint32_t nlo = D.z, ncount = D.w - D.z + 1u;
@@ -1307,9 +1323,9 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         if (act == 1) pl[np + __popcll(m1 & lt)] = P;
         if (QUAD && act == 2) {  // only accepted nodes fetch their moments
           const uint32_t q = nq + __popcll(m2 & lt);
-          const float4 q1 = s.quad[2 * ni + 1];
+          const float4 q1 = gload4(s.quad, 2 * (size_t)ni + 1);
           qa[q] = P;
-          qb[q] = s.quad[2 * ni];
+          qb[q] = gload4(s.quad, 2 * (size_t)ni);
           qc[q] = make_float2(q1.x, q1.y);
         }
         if (act == 3) {
