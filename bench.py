@@ -53,9 +53,9 @@ FLOPS_L2P = 60
 # by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
 # reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
 # written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 43.68e9 + 0.558e9
+WALK_TRAFFIC_BYTES = 43.57e9 + 0.558e9
 WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8)  # (leaf size, far_theta, mono_ratio) it was measured on
-WALK_VALU_ISSUE = 0.92  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
+WALK_VALU_ISSUE = 0.87  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
