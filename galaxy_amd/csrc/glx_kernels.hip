@@ -1281,9 +1281,11 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const float4 Df = gload4(nrec, nr4 + 3);
         const uint4 D = make_uint4(__float_as_uint(Df.x), __float_as_uint(Df.y),
                                    __float_as_uint(Df.z), __float_as_uint(Df.w));
-        const float ex = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
-        const float ey = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
-        const float ez = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
+        // com relative to the group centre; distance from the group box to the com
+        const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
+        const float ex = fmaxf(fabsf(dx) - hx, 0.f);
+        const float ey = fmaxf(fabsf(dy) - hy, 0.f);
+        const float ez = fmaxf(fabsf(dz) - hz, 0.f);
         const float dmin2 = ex * ex + ey * ey + ez * ez;
         // The boxes must be disjoint. Every point of the node's box lies within sqrt(3) s of
         // its com, and an accepted com is at least s / theta + delta from the group box, so
@@ -1302,7 +1304,6 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         bool far = false;
         if (FAR) {
           const float rn = __uint_as_float(D.y) + rg;
-          const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
           far = accept & (rn * rn <= s.far2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
         }
         // an accepted node that also passes the opening test at theta * mono_ratio is a
