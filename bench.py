@@ -236,6 +236,7 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
                  "quad_terms_per_particle": cnt["quad_terms"] / n_tot,
                  "node_tests_per_particle": cnt["node_tests"] / n_tot,
                  "far_nodes_per_particle": cnt["far_nodes"] / n_tot,
+                 "round_fill": cnt["node_tests"] / max(1, 64 * cnt["rounds"]),
                  "fallbacks": cnt["fallbacks"], "max_stack": cnt["max_stack"]},
         "error": quantiles(np.concatenate(errs)),
         "admit_s": t_admit,

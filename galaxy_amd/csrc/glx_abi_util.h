@@ -128,7 +128,7 @@ inline void sync(hipStream_t st) { check(hipStreamSynchronize(st), "hipStreamSyn
 inline void zero_walk_counters(const SegDesc& d, hipStream_t st) {
   check(hipMemsetAsync(d.counters + kCntInteractions, 0, 2 * sizeof(unsigned long long), st),
         "memset counters");
-  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 4 * sizeof(unsigned long long), st),
+  check(hipMemsetAsync(d.counters + kCntOverflow, 0, 5 * sizeof(unsigned long long), st),
         "memset counters");
 }
 
@@ -140,6 +140,7 @@ inline glx_stats stats_from(const unsigned long long* c) {
   s.fallbacks = c[kCntOverflow];
   s.max_stack = c[kCntMaxStack];
   s.far_nodes = c[kCntFar];
+  s.rounds = c[kCntRounds];
   return s;
 }
 

@@ -57,6 +57,7 @@ enum : int {
   kCntQuad = 4,          // quadrupole-node terms, per target
   kCntMaxStack = 5,      // deepest group-walk stack, entries
   kCntFar = 6,           // nodes folded into a group's local expansion, per group
+  kCntRounds = 7,        // group-walk rounds, summed over groups
   kCounters = 8
 };
 

@@ -1214,7 +1214,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     // that start in this chunk of sorted leaves, one after the other
     const uint32_t ge = lane < tl.count ? gbl(s.gend)[tl.start + lane] : kEnd;
     unsigned long long starts = __ballot(ge != kEnd);
-    unsigned long long t_int = 0, t_vis = 0, t_quad = 0, t_far = 0;
+    unsigned long long t_int = 0, t_vis = 0, t_quad = 0, t_far = 0, t_rounds = 0;
     uint32_t t_peak = 0;
     while (starts != 0ull) {
       const uint32_t l0 = (uint32_t)__builtin_ctzll(starts);
@@ -1247,7 +1247,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
 #pragma unroll
       for (int c = 0; c < (FAR ? kLocal : 1); ++c) loc[c] = 0.0f;
       Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      unsigned long long n_int = 0, n_vis = 0, n_quad = 0, n_far = 0;
+      unsigned long long n_int = 0, n_vis = 0, n_quad = 0, n_far = 0, n_rounds = 0;
       uint32_t sp = 1, np = 0, nq = 0, nr = 0, nf = 0, peak = 1;
       bool overflow = false;
       if (lane == 0) stk[0] = (n == 1) ? 0u : n;
@@ -1263,7 +1263,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const uint32_t base = sp - m;
         const uint32_t u = lane < m ? stk[base + lane] : kEnd;
         sp = base;
-        if (COUNT && lane == 0) n_vis += m;
+        if (COUNT && lane == 0) {
+          n_vis += m;
+          ++n_rounds;
+        }
         // Classify without divergent branches: every lane loads a valid leaf and node
         // record (index 0 stands in for the other kind) and the action is selected, so
         // the ballots below always see the full wave.
@@ -1405,6 +1408,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         t_quad += wave_sum_u64(valid ? n_quad : 0ull);
         t_vis += wave_sum_u64(n_vis);
         t_far += wave_sum_u64(n_far);
+        t_rounds += wave_sum_u64(n_rounds);
         t_peak = peak > t_peak ? peak : t_peak;
       }
       __syncthreads();
@@ -1415,6 +1419,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       atomicAdd(&s.counters[kCntVisits], t_vis);
       atomicMax(&s.counters[kCntMaxStack], (unsigned long long)t_peak);
       atomicAdd(&s.counters[kCntFar], t_far);
+      atomicAdd(&s.counters[kCntRounds], t_rounds);
     }
   }
 }

@@ -238,6 +238,7 @@ extern "C" int glx_pool_count(glx_pool* p, glx_stats* out) {
       out->fallbacks += s.fallbacks;
       out->max_stack = s.max_stack > out->max_stack ? s.max_stack : out->max_stack;
       out->far_nodes += s.far_nodes;
+      out->rounds += s.rounds;
     }
   });
 }

@@ -67,6 +67,7 @@ class Stats(C.Structure):
         ("fallbacks", C.c_uint64),
         ("max_stack", C.c_uint64),
         ("far_nodes", C.c_uint64),
+        ("rounds", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -88,6 +88,7 @@ typedef struct glx_stats {
   uint64_t fallbacks;   /* group walks that fell back to the per-target walk */
   uint64_t max_stack;   /* deepest group-walk node stack, entries */
   uint64_t far_nodes;   /* nodes folded into group local expansions (per group) */
+  uint64_t rounds;      /* group-walk rounds (up to 64 node tests each), summed over groups */
 } glx_stats;
 
 /* Message of the last failed call on this thread ("" if none). */
