@@ -1,7 +1,7 @@
 """End-to-end GPU forces (the ForceSolver path) against the oracle.
 
-Ports the reference's GPU gates (gpu/tests/gpu_lbvh.rs) and adds the comparison the
-north star asks for: the same-theta reference CPU path (oracle) and an FP64 direct sum,
+Ports the reference's GPU gates (gpu/tests/gpu_lbvh.rs) and adds two comparisons: the
+same-theta reference CPU path (oracle) and an FP64 direct sum,
 each as a median and 99th-percentile per-particle relative error with the tolerances
 written below. Because Morton codes, sort and topology are bit-exact, the same-theta
 difference is f32 arithmetic plus rare opening-test flips near the acceptance boundary.
