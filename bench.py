@@ -54,7 +54,9 @@ FLOPS_L2P = 60
 # reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
 # written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
 WALK_TRAFFIC_BYTES = 43.57e9 + 0.558e9
-WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8)  # (leaf size, far_theta, mono_ratio) it was measured on
+# (leaf size, far_theta, mono_ratio, group size, lanes per body) the walk counters were
+# measured on
+WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4)
 WALK_VALU_ISSUE = 0.87  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
@@ -512,6 +514,12 @@ def main():
                     help="far-field ratio of the group walk (0: off; negative: autotune)")
     ap.add_argument("--mono-ratio", type=float, default=-1.0,
                     help="monopole band of the group walk (0: off; negative: autotune)")
+    ap.add_argument("--group-size", type=int, default=0,
+                    help="bodies per walk group, 8..64 (0: autotune)")
+    ap.add_argument("--group-lanes", type=int, default=0,
+                    help="most lanes per body in groups of <= 32 bodies, 1..4 (0: autotune)")
+    ap.add_argument("--radix-bits", type=int, default=0,
+                    help="sort digit width, 8, 9 or 10 (0: autotune)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
     ap.add_argument("--serve-device-ic", action="store_true",
                     help="serving jobs drawn on the device at admission (glx_pool_admit_ic)")
@@ -576,21 +584,30 @@ def main():
         f"ICs {time.perf_counter() - t0:.1f}s")
     tuning = None
     leaf, far, mono = args.leaf_size, max(args.far_theta, 0.0), max(args.mono_ratio, 0.0)
-    if not (args.leaf_size and args.far_theta >= 0 and args.mono_ratio >= 0):
-        from galaxy_amd.tune import tune_leaf_size
+    gsize, glanes, rbits = args.group_size or 64, args.group_lanes or 4, args.radix_bits or 8
+    if not (args.leaf_size and args.far_theta >= 0 and args.mono_ratio >= 0 and
+            args.group_size and args.group_lanes and args.radix_bits):
+        from galaxy_amd.tune import GROUP_SHAPES, tune
 
         pick = np.random.default_rng(args.seed + 7).choice(len(systems), 64, replace=False)
+        shapes = [(g, n) for g, n in GROUP_SHAPES
+                  if args.group_size in (0, g) and args.group_lanes in (0, n)] or [(gsize, glanes)]
         t0 = time.perf_counter()
-        tuning = tune_leaf_size([systems[k] for k in sorted(pick)], softening=eps, theta=theta,
-                                dt=dt, leaf_sizes=(args.leaf_size,) if args.leaf_size else None,
-                                far_thetas=(args.far_theta,) if args.far_theta >= 0 else None,
-                                mono_ratios=(args.mono_ratio,) if args.mono_ratio >= 0 else None)
+        tuning = tune([systems[k] for k in sorted(pick)], softening=eps, theta=theta, dt=dt,
+                      leaf_sizes=(args.leaf_size,) if args.leaf_size else None,
+                      far_thetas=(args.far_theta,) if args.far_theta >= 0 else None,
+                      mono_ratios=(args.mono_ratio,) if args.mono_ratio >= 0 else None,
+                      group_shapes=shapes,
+                      radix_widths=(args.radix_bits,) if args.radix_bits else None,
+                      radix_systems=systems)
         leaf, far, mono = tuning["leaf_size"], tuning["far_theta"], tuning["mono_ratio"]
+        gsize, glanes, rbits = tuning["group_size"], tuning["group_lanes"], tuning["radix_bits"]
         log(f"tuned in {time.perf_counter() - t0:.1f}s: leaf size {leaf}, far_theta {far}, "
-            f"mono_ratio {mono}: {tuning['table']}")
-    fast_mode["leaf_size"] = leaf
-    fast_mode["far_theta"] = far
-    fast_mode["mono_ratio"] = mono
+            f"mono_ratio {mono}, group {gsize}x{glanes} lanes, radix {rbits} bits: "
+            f"{tuning['table']} {tuning['shape_table']} {tuning['radix_table']}")
+    fast_mode.update(leaf_size=leaf, far_theta=far, mono_ratio=mono, group_size=gsize,
+                     group_lanes=glanes, radix_bits=rbits)
+    knobs = (leaf, far, mono, gsize, glanes)
     b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, fast_mode, 8)
     log(f"batched: {b['value']:.4g} pf/s, {b['ms_per_step']:.2f} ms/step, err {b['error']}")
     out = {
@@ -611,7 +628,8 @@ def main():
                    "systems": len(systems), "particles": int(sizes.sum()),
                    "n_range": [1024, 65536], "theta": theta, "eps": eps, "dt": dt,
                    "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
-                   "leaf_size": leaf, "far_theta": far, "mono_ratio": mono},
+                   "leaf_size": leaf, "far_theta": far, "mono_ratio": mono,
+                   "group_size": gsize, "group_lanes": glanes, "radix_bits": rbits},
         "p50_step_ms": b["device_ms"]["p50"],
         "p99_step_ms": b["device_ms"]["p99"],
         "force_error_vs_fp64_direct": b["error"],
@@ -630,20 +648,20 @@ def main():
             # 150 per far node, 60 per body of a local expansion)
             "achieved_executed": b["walk"]["tflops"],
             "frac_executed": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
-            "traffic": WALK_TRAFFIC_BYTES if (leaf, far, mono) == WALK_TRAFFIC_CONFIG else None,
+            "traffic": WALK_TRAFFIC_BYTES if knobs == WALK_TRAFFIC_CONFIG else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
-                              " leaf size 1, far-field ratio 0.55, monopole band 0.8",
+                              " leaf size 1, far-field ratio 0.55, monopole band 0.8, groups"
+                              " of 64, 4 lanes",
             "flops_per_launch": b["walk"]["flops_ref"],
             "executed_flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
             # VALU wave-instructions issued per launch / what 256 CUs issue in its duration
             # (rocprofv3 SQ_INSTS_VALU, profiles/sq_counters_walk_ensemble512.md): the
             # walk is issue-bound although its counted flops are a small fraction of peak
-            "valu_issue_frac": (WALK_VALU_ISSUE if (leaf, far, mono) == WALK_TRAFFIC_CONFIG
-                                else None),
+            "valu_issue_frac": WALK_VALU_ISSUE if knobs == WALK_TRAFFIC_CONFIG else None,
         },
         "batched": strip(b),
-        "leaf_size_tuning": tuning,
+        "tuning": tuning,
     }
     if not args.no_plain:
         p = plain(glx, systems, eps, theta, dt, args.plain_steps, fast_mode)

@@ -62,6 +62,11 @@ inline void validate_params(const glx_params* p) {
     fail(GLX_EINVAL, "mono_ratio must be in [0, 1)");
   if (p->mono_ratio > 0.0 && p->multipole != 2)
     fail(GLX_EINVAL, "mono_ratio needs quadrupoles");
+  if (p->radix_bits != 0 && (p->radix_bits < 8 || p->radix_bits > (uint32_t)kRadixBitsMax))
+    fail(GLX_EINVAL, "radix_bits must be 8, 9 or 10");
+  if (p->group_size != 0 && (p->group_size < 8 || p->group_size > (uint32_t)kGroup))
+    fail(GLX_EINVAL, "group_size must be in [8, 64]");
+  if (p->group_lanes > 4) fail(GLX_EINVAL, "group_lanes must be in [1, 4]");
 }
 
 inline Mode mode_from(const glx_params& p) {
@@ -71,6 +76,7 @@ inline Mode mode_from(const glx_params& p) {
   m.quad = p.multipole == 2;
   m.kahan = p.precision == GLX_PREC_KAHAN;
   m.far = p.far_theta > 0.0;
+  m.radix_bits = p.radix_bits ? (int)p.radix_bits : kRadixBits;
   return m;
 }
 
@@ -85,6 +91,8 @@ inline void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
   d.theta = (float)p.theta;
   d.precision = p.precision;
   d.leaf_size = p.leaf_size ? p.leaf_size : 1u;
+  d.group_size = p.group_size ? p.group_size : (uint32_t)kGroup;
+  d.group_lanes = p.group_lanes ? p.group_lanes : 4u;
   d.far2 = (float)(p.far_theta * p.far_theta);
   d.mono_inv = p.mono_ratio > 0.0 ? (float)(1.0 / (p.theta * p.mono_ratio)) : 0.0f;
 }

@@ -317,10 +317,12 @@ struct StageCtx {
   Stream S;
   Segment seg;
   Batch batch;
-  explicit StageCtx(bool key63 = false, bool quad = false) : batch(S.st) {
+  explicit StageCtx(bool key63 = false, bool quad = false, int radix_bits = kRadixBits)
+      : batch(S.st) {
     Mode m;
     m.key63 = key63;
     m.quad = quad;
+    m.radix_bits = radix_bits;
     batch.set_mode(m);
   }
   void describe(uint32_t n) {
@@ -356,14 +358,22 @@ void stage_morton(size_t n, const double* pos, double* bounds_out, uint32_t* lan
 }
 
 template <class K>
-void stage_sort(size_t n, const K* codes, uint32_t* order, K* sorted_codes) {
+void stage_sort(size_t n, const K* codes, uint32_t* order, K* sorted_codes,
+                uint32_t radix_bits = kRadixBits) {
   validate_n(n);
+  if (radix_bits < 8 || radix_bits > (uint32_t)kRadixBitsMax)
+    fail(GLX_EINVAL, "radix_bits must be 8, 9 or 10");
   if (n == 0) return;
   if (!codes) fail(GLX_EINVAL, "NULL array");
+  constexpr bool key63 = sizeof(K) == 8;
+  const int span = sort_passes_for(key63, (int)radix_bits) * (int)radix_bits;
+  if (span < (int)(8 * sizeof(K)))
+    for (size_t k = 0; k < n; ++k)
+      if (codes[k] >> span) fail(GLX_EINVAL, "a code exceeds the sort passes' bits");
   ensure_device();
-  StageCtx c(sizeof(K) == 8);
+  StageCtx c(key63, false, (int)radix_bits);
   c.describe((uint32_t)n);
-  h2d(c.seg.desc.keys[0], codes, n * sizeof(K), c.S.st);
+  h2d(c.seg.desc.keys[sort_start(key63, (int)radix_bits)], codes, n * sizeof(K), c.S.st);
   c.batch.iota();
   c.batch.sort();
   if (order) d2h(order, c.seg.desc.vals[0], n * sizeof(uint32_t), c.S.st);
@@ -414,6 +424,16 @@ extern "C" int glx_stage_sort(size_t n, const uint32_t* codes, uint32_t* order,
 extern "C" int glx_stage_sort63(size_t n, const uint64_t* codes, uint32_t* order,
                                 uint64_t* sorted_codes) {
   return guard([&] { stage_sort<uint64_t>(n, codes, order, sorted_codes); });
+}
+
+extern "C" int glx_stage_sort_bits(size_t n, const uint32_t* codes, uint32_t* order,
+                                   uint32_t* sorted_codes, uint32_t radix_bits) {
+  return guard([&] { stage_sort<uint32_t>(n, codes, order, sorted_codes, radix_bits); });
+}
+
+extern "C" int glx_stage_sort63_bits(size_t n, const uint64_t* codes, uint32_t* order,
+                                     uint64_t* sorted_codes, uint32_t radix_bits) {
+  return guard([&] { stage_sort<uint64_t>(n, codes, order, sorted_codes, radix_bits); });
 }
 
 extern "C" int glx_stage_karras(size_t n, const uint32_t* sorted_codes, uint32_t* left,

@@ -18,8 +18,9 @@ namespace glx {
 
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;  // solvers/src/lbvh.rs:244
 constexpr uint32_t kEnd = 0xFFFFFFFFu;       // walk terminator
-constexpr int kRadixBits = 8;
-constexpr int kRadixBins = 1 << kRadixBits;
+constexpr int kRadixBits = 8;                          // default sort digit width
+constexpr int kRadixBitsMax = 10;                      // widest digit (8, 9 or 10 bits)
+constexpr int kRadixBinsMax = 1 << kRadixBitsMax;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per element tile
@@ -44,7 +45,17 @@ struct Mode {
   bool quad = false;     // quadrupole moments (group walk only)
   bool kahan = false;    // Kahan-compensated accumulation
   bool far = false;      // per-group local expansion of well-separated nodes (group walk)
+  int radix_bits = kRadixBits;  // sort digit width: 8, 9 or 10 bits per pass
 };
+
+// Passes of the LSD sort: key bits over the digit width. An odd count starts the sort in
+// buffer 1 so that the sorted keys and payload always end in buffer 0.
+__host__ __device__ constexpr int sort_passes_for(bool key63, int radix_bits) {
+  return ((key63 ? 63 : 30) + radix_bits - 1) / radix_bits;
+}
+__host__ __device__ constexpr int sort_start(bool key63, int radix_bits) {
+  return sort_passes_for(key63, radix_bits) & 1;
+}
 
 enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1 };
 
@@ -92,6 +103,8 @@ struct SegDesc {
   uint32_t sort_tile0, sort_tiles;  // element tiles (kSortTile) of this segment
   uint32_t precision;
   uint32_t leaf_size;  // group walk: nodes of at most this many bodies are summed body by body
+  uint32_t group_size;  // group walk: bodies per walk group (subtrees of at most this many)
+  uint32_t group_lanes; // group walk: most lanes per body in groups of at most 32 bodies
   float g, eps2, theta, half_dt;
   float far2;  // squared far-field ratio: (node radius + group radius)^2 <= far2 * R^2
   float mono_inv;  // 1 / (theta mono_ratio): accepted nodes this far are monopoles (0: off)

@@ -13,7 +13,8 @@ hipError_t launch_kick_drift_bbox(hipStream_t st, const SegDesc* segs, const Til
 hipError_t launch_encode(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net, uint32_t* lanes_out);
 hipError_t launch_bounds_only(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg);
-hipError_t launch_iota(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
+hipError_t launch_iota(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
+                       uint32_t net);
 hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
                        const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off);
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);

@@ -126,7 +126,6 @@ struct KeyTraits;
 template <>
 struct KeyTraits<uint32_t> {
   static constexpr int cell_bits = 10;  // solvers/src/lbvh.rs:32
-  static constexpr int passes = 4;
   static constexpr int tie_base = 32;
   __device__ static uint32_t code(uint32_t x, uint32_t y, uint32_t z) {
     return expand10(x) | (expand10(y) << 1) | (expand10(z) << 2);
@@ -136,7 +135,6 @@ struct KeyTraits<uint32_t> {
 template <>
 struct KeyTraits<uint64_t> {
   static constexpr int cell_bits = 21;  // the 63-bit refinement of solvers/src/lbvh.rs:29-31
-  static constexpr int passes = 8;
   static constexpr int tie_base = 64;
   __device__ static uint64_t code(uint32_t x, uint32_t y, uint32_t z) {
     return expand21(x) | (expand21(y) << 1) | (expand21(z) << 2);
@@ -234,7 +232,7 @@ k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
 template <class K>
 __global__ void __launch_bounds__(kBlock)
 k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
-         uint32_t* __restrict__ lanes_out) {
+         uint32_t* __restrict__ lanes_out, int dst) {
 #pragma clang fp contract(off)
   using T = KeyTraits<K>;
   const double top = (double)((1u << T::cell_bits) - 1u);
@@ -252,7 +250,8 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       s.bounds[6] = b.origin[1];
       s.bounds[7] = b.origin[2];
     }
-    K* keys = static_cast<K*>(s.keys[0]);
+    K* keys = static_cast<K*>(s.keys[dst]);  // the sort's first source buffer
+    uint32_t* vals = s.vals[dst];
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const uint32_t i = tl.start + e;
       uint32_t l[3];
@@ -264,7 +263,7 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
         lanes_out[3 * (size_t)i + 2] = l[2];
       }
       keys[i] = T::code(l[0], l[1], l[2]);
-      s.vals[0][i] = i;
+      vals[i] = i;
     }
   }
 }
@@ -284,74 +283,78 @@ __global__ void k_bounds(const SegDesc* __restrict__ segs, uint32_t nseg, int ce
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_iota(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
+k_iota(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
+       int dst) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock)
-      s.vals[0][tl.start + e] = tl.start + e;
+      s.vals[dst][tl.start + e] = tl.start + e;
   }
 }
 
 // ----------------------------------------------------------------------------------
 // 3. stable LSD radix sort, segmented reduce-then-scan (per pass: upsweep, scan,
-//    downsweep). Pure integer; reproduces the (code, index) order of
-//    solvers/src/lbvh.rs:143-147 because the payload starts as iota and every pass
-//    is stable.
+//    downsweep) with B-bit digits (B = 8, 9 or 10: 2^B bins). Pure integer; reproduces
+//    the (code, index) order of solvers/src/lbvh.rs:143-147 because the payload starts
+//    as iota and every pass is stable, whatever the digit width.
 // ----------------------------------------------------------------------------------
 
-template <class K>
+template <class K, int B>
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_upsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
                 uint32_t* __restrict__ tile_hist, int shift, int src) {
-  __shared__ uint32_t hist[kRadixBins];
+  constexpr int BINS = 1 << B;
+  __shared__ uint32_t hist[BINS];
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const K* keys = static_cast<const K*>(s.keys[src]) + tl.start;
-    hist[threadIdx.x] = 0;
+    for (int d = threadIdx.x; d < BINS; d += kSortThreads) hist[d] = 0;
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < tl.count; e += kSortThreads)
-      atomicAdd(&hist[(uint32_t)(keys[e] >> shift) & (kRadixBins - 1)], 1u);
+      atomicAdd(&hist[(uint32_t)(keys[e] >> shift) & (BINS - 1)], 1u);
     __syncthreads();
-    tile_hist[(size_t)t * kRadixBins + threadIdx.x] = hist[threadIdx.x];
+    for (int d = threadIdx.x; d < BINS; d += kSortThreads)
+      tile_hist[(size_t)t * BINS + d] = hist[d];
     __syncthreads();
   }
 }
 
-// One block of 1024 threads per segment: thread (digit d, chunk g) scans a quarter of
-// the segment's tiles; tile offsets = digit base (exclusive scan over digits of the
-// digit totals) + exclusive prefix over earlier tiles.
+// One block of 1024 threads per segment: thread (digit d, chunk g) scans one of the
+// 1024 / 2^B chunks of the segment's tiles; tile offsets = digit base (exclusive scan
+// over digits of the digit totals) + exclusive prefix over earlier tiles.
 constexpr int kScanThreads = 1024;
-constexpr int kScanChunks = kScanThreads / kRadixBins;
 
+template <int B>
 __global__ void __launch_bounds__(kScanThreads)
 k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg,
              const uint32_t* __restrict__ tile_hist,
              uint32_t* __restrict__ tile_off) {
-  __shared__ uint32_t part[kScanChunks][kRadixBins];
-  __shared__ uint32_t base[kRadixBins];
-  __shared__ uint32_t wsum[kRadixBins / 64];
-  const uint32_t d = threadIdx.x & (kRadixBins - 1), g = threadIdx.x / kRadixBins;
+  constexpr int BINS = 1 << B, CHUNKS = kScanThreads / BINS;
+  __shared__ uint32_t part[CHUNKS][BINS];
+  __shared__ uint32_t base[BINS];
+  __shared__ uint32_t wsum[BINS / 64];
+  const uint32_t d = threadIdx.x & (BINS - 1), g = threadIdx.x / BINS;
   for (uint32_t sidx = blockIdx.x; sidx < nseg; sidx += gridDim.x) {
     const SegDesc& s = segs[sidx];
-    const uint32_t nt = s.sort_tiles, per = (nt + kScanChunks - 1) / kScanChunks;
+    const uint32_t nt = s.sort_tiles, per = (nt + CHUNKS - 1) / CHUNKS;
     const uint32_t c0 = s.sort_tile0 + min(nt, g * per), c1 = s.sort_tile0 + min(nt, (g + 1) * per);
     uint32_t sum = 0;
 #pragma unroll 8
-    for (uint32_t t = c0; t < c1; ++t) sum += tile_hist[(size_t)t * kRadixBins + d];
+    for (uint32_t t = c0; t < c1; ++t) sum += tile_hist[(size_t)t * BINS + d];
     part[g][d] = sum;
     __syncthreads();
-    if (threadIdx.x < kRadixBins) {
+    if (threadIdx.x < BINS) {
       uint32_t tot = 0;
-      for (int q = 0; q < kScanChunks; ++q) tot += part[q][d];
-      const int lane = d & 63, w = d >> 6;
+      for (int q = 0; q < CHUNKS; ++q) tot += part[q][d];
+      const int lane = d & 63;
       uint32_t inc = tot;
       for (int o = 1; o < 64; o <<= 1) {
         const uint32_t v = __shfl_up(inc, o);
         if (lane >= o) inc += v;
       }
-      if (lane == 63) wsum[w] = inc;
+      if (lane == 63) wsum[d >> 6] = inc;
       base[d] = inc - tot;
     }
     __syncthreads();
@@ -360,22 +363,24 @@ k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg,
     for (uint32_t q = 0; q < g; ++q) run += part[q][d];
 #pragma unroll 8
     for (uint32_t t = c0; t < c1; ++t) {
-      const uint32_t h = tile_hist[(size_t)t * kRadixBins + d];
-      tile_off[(size_t)t * kRadixBins + d] = run;
+      const uint32_t h = tile_hist[(size_t)t * BINS + d];
+      tile_off[(size_t)t * BINS + d] = run;
       run += h;
     }
     __syncthreads();
   }
 }
 
-template <class K>
+template <class K, int B>
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
                   const uint32_t* __restrict__ tile_off, int shift, int src) {
+  constexpr int BINS = 1 << B;
   constexpr int W = kSortThreads / 64;
   constexpr int PER_WAVE = kSortTile / W;
-  __shared__ uint32_t wcnt[W][kRadixBins];
-  __shared__ uint32_t dstart[kRadixBins];
+  constexpr int R = BINS / kSortThreads;  // digits per thread in the tile scan
+  __shared__ uint32_t wcnt[W][BINS];
+  __shared__ uint32_t dstart[BINS];
   __shared__ K skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   __shared__ uint32_t shw[W];
@@ -388,7 +393,8 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
     const uint32_t* vals_in = s.vals[src] + tl.start;
     K* keys_out = static_cast<K*>(s.keys[src ^ 1]);
     uint32_t* vals_out = s.vals[src ^ 1];
-    for (int k = 0; k < W; ++k) wcnt[k][threadIdx.x] = 0;
+    for (int k = 0; k < W; ++k)
+      for (int d = threadIdx.x; d < BINS; d += kSortThreads) wcnt[k][d] = 0;
     __syncthreads();
 
     K key[kSortItems];
@@ -399,12 +405,11 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
       const bool valid = e < tl.count;
       key[j] = valid ? keys_in[e] : (K)0;
       val[j] = valid ? vals_in[e] : 0u;
-      const uint32_t dd =
-          valid ? ((uint32_t)(key[j] >> shift) & (kRadixBins - 1)) : (uint32_t)kRadixBins;
-      // lanes holding the same digit (bit kRadixBits separates the invalid tail)
+      const uint32_t dd = valid ? ((uint32_t)(key[j] >> shift) & (BINS - 1)) : (uint32_t)BINS;
+      // lanes holding the same digit (bit B separates the invalid tail)
       unsigned long long peers = ~0ull;
 #pragma unroll
-      for (int b = 0; b <= kRadixBits; ++b) {
+      for (int b = 0; b <= B; ++b) {
         const bool bit = (dd >> b) & 1u;
         const unsigned long long bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -420,33 +425,41 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
     }
     __syncthreads();
     {
-      const uint32_t d = threadIdx.x;
-      uint32_t c[W], tot = 0;
-      for (int k = 0; k < W; ++k) {
-        c[k] = wcnt[k][d];
-        tot += c[k];
+      // thread q owns digits [q R, q R + R): per-wave starts, then the tile's digit starts
+      uint32_t tot[R], sum = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int d = threadIdx.x * R + r;
+        uint32_t run = 0;
+        for (int k = 0; k < W; ++k) {
+          const uint32_t c = wcnt[k][d];
+          wcnt[k][d] = run;
+          run += c;
+        }
+        tot[r] = run;
+        sum += run;
       }
-      uint32_t run = 0;
-      for (int k = 0; k < W; ++k) {
-        wcnt[k][d] = run;
-        run += c[k];
+      uint32_t first = block_exclusive_scan(sum, shw);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        dstart[threadIdx.x * R + r] = first;
+        first += tot[r];
       }
-      dstart[d] = block_exclusive_scan(tot, shw);
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-      if (dig[j] < (uint32_t)kRadixBins) {
+      if (dig[j] < (uint32_t)BINS) {
         const uint32_t p = dstart[dig[j]] + wcnt[w][dig[j]] + rank[j];
         skey[p] = key[j];
         sval[p] = val[j];
       }
     }
     __syncthreads();
-    const uint32_t* off = tile_off + (size_t)t * kRadixBins;
+    const uint32_t* off = tile_off + (size_t)t * BINS;
     for (uint32_t p = threadIdx.x; p < tl.count; p += kSortThreads) {
       const K k = skey[p];
-      const uint32_t d = (uint32_t)(k >> shift) & (kRadixBins - 1);
+      const uint32_t d = (uint32_t)(k >> shift) & (BINS - 1);
       const uint32_t dst = off[d] + (p - dstart[d]);
       keys_out[dst] = k;
       vals_out[dst] = sval[p];
@@ -562,11 +575,12 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       s.lo[i] = (uint32_t)lo;
       s.hi[i] = (uint32_t)hi;
       if (i == 0) s.parent_int[0] = kNoParent;
-      // walk groups: the subtrees of at most kGroup bodies whose parent holds more;
-      // a node above the limit marks those of its two children
-      if (hi - lo + 1 > kGroup) {
-        if (gamma - lo + 1 <= kGroup) s.gend[lo] = (uint32_t)gamma;
-        if (hi - gamma <= kGroup) s.gend[gamma + 1] = (uint32_t)hi;
+      // walk groups: the subtrees of at most group_size (<= 64) bodies whose parent holds
+      // more; a node above the limit marks those of its two children
+      const int gmax = (int)s.group_size;
+      if (hi - lo + 1 > gmax) {
+        if (gamma - lo + 1 <= gmax) s.gend[lo] = (uint32_t)gamma;
+        if (hi - gamma <= gmax) s.gend[gamma + 1] = (uint32_t)hi;
       } else if (i == 0) {
         s.gend[0] = (uint32_t)(n - 1);
       }
@@ -1225,7 +1239,9 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
       // A group of at most 32 bodies gives each body rep lanes (2, 3 or 4): the lists are
       // drained with stride rep, and the partial sums are folded into the body's first
       // lane (its owner) before the store.
-      const uint32_t rep = gcount > 32u ? 1u : (gcount > 21u ? 2u : (gcount > 16u ? 3u : 4u));
+      // (at most group_lanes; rep >= 1 also lets the compiler count the drains' trips)
+      const uint32_t rep = max(min(gcount > 32u ? 1u : (gcount > 21u ? 2u : (gcount > 16u ? 3u : 4u)),
+                                   s.group_lanes), 1u);
       const uint32_t sub = (uint32_t)(lane >= gcount) + (uint32_t)(lane >= 2u * gcount) +
                            (uint32_t)(lane >= 3u * gcount);
       const bool owner = lane < gcount;
@@ -1616,12 +1632,13 @@ hipError_t launch_kick_drift_bbox(hipStream_t st, const SegDesc* segs, const Til
 
 hipError_t launch_encode(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net, uint32_t* lanes_out) {
+  const int dst = sort_start(md.key63, md.radix_bits);
   if (md.key63)
     hipLaunchKernelGGL(k_encode<uint64_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net,
-                       lanes_out);
+                       lanes_out, dst);
   else
     hipLaunchKernelGGL(k_encode<uint32_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net,
-                       lanes_out);
+                       lanes_out, dst);
   return hipGetLastError();
 }
 
@@ -1631,33 +1648,49 @@ hipError_t launch_bounds_only(hipStream_t st, const Mode& md, const SegDesc* seg
   return hipGetLastError();
 }
 
-hipError_t launch_iota(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_iota, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+hipError_t launch_iota(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
+                       uint32_t net) {
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net,
+                     sort_start(md.key63, md.radix_bits));
   return hipGetLastError();
 }
 
-template <class K>
+// The keys start in buffer sort_start() and end in buffer 0.
+template <class K, int B>
 static void sort_passes(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
                         uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
-  int src = 0;
-  for (int pass = 0; pass < KeyTraits<K>::passes; ++pass) {
-    const int shift = pass * kRadixBits;
-    hipLaunchKernelGGL(k_radix_upsweep<K>, dim3(grid_for(net)), dim3(kSortThreads), 0, st, segs,
-                       et, net, tile_hist, shift, src);
-    hipLaunchKernelGGL(k_radix_scan, dim3(grid_for(nseg)), dim3(kScanThreads), 0, st, segs, nseg,
-                       tile_hist, tile_off);
-    hipLaunchKernelGGL(k_radix_downsweep<K>, dim3(grid_for(net)), dim3(kSortThreads), 0, st, segs,
-                       et, net, tile_off, shift, src);
+  constexpr bool key63 = sizeof(K) == 8;
+  int src = sort_start(key63, B);
+  for (int pass = 0; pass < sort_passes_for(key63, B); ++pass) {
+    const int shift = pass * B;
+    hipLaunchKernelGGL((k_radix_upsweep<K, B>), dim3(grid_for(net)), dim3(kSortThreads), 0, st,
+                       segs, et, net, tile_hist, shift, src);
+    hipLaunchKernelGGL(k_radix_scan<B>, dim3(grid_for(nseg)), dim3(kScanThreads), 0, st, segs,
+                       nseg, tile_hist, tile_off);
+    hipLaunchKernelGGL((k_radix_downsweep<K, B>), dim3(grid_for(net)), dim3(kSortThreads), 0, st,
+                       segs, et, net, tile_off, shift, src);
     src ^= 1;
   }
 }
 
+template <class K>
+static void sort_width(hipStream_t st, int bits, const SegDesc* segs, uint32_t nseg,
+                       const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+  if (bits == 10)
+    sort_passes<K, 10>(st, segs, nseg, et, net, tile_hist, tile_off);
+  else if (bits == 9)
+    sort_passes<K, 9>(st, segs, nseg, et, net, tile_hist, tile_off);
+  else
+    sort_passes<K, 8>(st, segs, nseg, et, net, tile_hist, tile_off);
+}
+
 hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
                        const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+  if (md.radix_bits < 8 || md.radix_bits > kRadixBitsMax) return hipErrorInvalidValue;
   if (md.key63)
-    sort_passes<uint64_t>(st, segs, nseg, et, net, tile_hist, tile_off);
+    sort_width<uint64_t>(st, md.radix_bits, segs, nseg, et, net, tile_hist, tile_off);
   else
-    sort_passes<uint32_t>(st, segs, nseg, et, net, tile_hist, tile_off);
+    sort_width<uint32_t>(st, md.radix_bits, segs, nseg, et, net, tile_hist, tile_off);
   return hipGetLastError();
 }
 

@@ -217,8 +217,8 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
     (void)hipFree(diag_partial_);
     tile_hist_ = tile_off_ = nullptr;
     diag_partial_ = nullptr;
-    check(hipMalloc(&tile_hist_, ce * kRadixBins * sizeof(uint32_t)), "hipMalloc(hist)");
-    check(hipMalloc(&tile_off_, ce * kRadixBins * sizeof(uint32_t)), "hipMalloc(hist)");
+    check(hipMalloc(&tile_hist_, ce * kRadixBinsMax * sizeof(uint32_t)), "hipMalloc(hist)");
+    check(hipMalloc(&tile_off_, ce * kRadixBinsMax * sizeof(uint32_t)), "hipMalloc(hist)");
     check(hipMalloc(&diag_partial_, ce * kDiagRecord * sizeof(double)), "hipMalloc(diag)");
   }
   cap_et_ = ce;
@@ -267,7 +267,7 @@ void Batch::encode(uint32_t* lanes_out) {
   check(launch_encode(stream_, mode_, segs_, etiles_, n_et_, lanes_out), "encode");
 }
 void Batch::bounds_only() { check(launch_bounds_only(stream_, mode_, segs_, nseg_), "bounds"); }
-void Batch::iota() { check(launch_iota(stream_, segs_, etiles_, n_et_), "iota"); }
+void Batch::iota() { check(launch_iota(stream_, mode_, segs_, etiles_, n_et_), "iota"); }
 void Batch::sort() {
   check(launch_sort(stream_, mode_, segs_, nseg_, etiles_, n_et_, tile_hist_, tile_off_), "sort");
 }

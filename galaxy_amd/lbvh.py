@@ -45,6 +45,9 @@ class Params(C.Structure):
         ("walk", C.c_uint32),
         ("far_theta", C.c_double),
         ("mono_ratio", C.c_double),
+        ("radix_bits", C.c_uint32),
+        ("group_size", C.c_uint32),
+        ("group_lanes", C.c_uint32),
     ]
 
 
@@ -110,6 +113,8 @@ def lib():
         "glx_stage_morton63": (I, [S, P, P, P, P]),
         "glx_stage_sort": (I, [S, P, P, P]),
         "glx_stage_sort63": (I, [S, P, P, P]),
+        "glx_stage_sort_bits": (I, [S, P, P, P, C.c_uint32]),
+        "glx_stage_sort63_bits": (I, [S, P, P, P, C.c_uint32]),
         "glx_stage_karras": (I, [S, P, P, P, P]),
         "glx_stage_karras63": (I, [S, P, P, P, P]),
         "glx_stage_aggregate": (I, [S, P, P, P, P, P, P, P, P, P]),
@@ -163,10 +168,11 @@ def device_info():
 
 def _params(g, softening, theta, precision=PREC_F32, count=False, morton_bits=30,
             walk=WALK_REFERENCE, multipole=0, leaf_size=1, far_theta=0.0,
-            mono_ratio=0.0) -> Params:
+            mono_ratio=0.0, radix_bits=0, group_size=0, group_lanes=0) -> Params:
     return Params(float(g), float(softening), float(theta), int(leaf_size), int(multipole),
                   int(precision), FLAG_COUNT if count else 0, int(morton_bits), int(walk),
-                  float(far_theta), float(mono_ratio))
+                  float(far_theta), float(mono_ratio), int(radix_bits), int(group_size),
+                  int(group_lanes))
 
 
 @dataclass
@@ -189,11 +195,12 @@ class Lbvh:
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
                  count: bool = False, morton_bits: int = 30, walk: int = WALK_REFERENCE,
                  multipole: int = 0, leaf_size: int = 1, far_theta: float = 0.0,
-                 mono_ratio: float = 0.0):
+                 mono_ratio: float = 0.0, radix_bits: int = 0, group_size: int = 0,
+                 group_lanes: int = 0):
         self.g, self.softening, self.theta = float(g), float(softening), float(theta)
         h = C.c_void_p()
         p = _params(g, softening, theta, precision, count, morton_bits, walk, multipole,
-                    leaf_size, far_theta, mono_ratio)
+                    leaf_size, far_theta, mono_ratio, radix_bits, group_size, group_lanes)
         _check(lib().glx_solver_create(C.byref(p), C.byref(h)))
         self._h = h
 
@@ -302,19 +309,25 @@ def morton(pos, bits: int = 30) -> MortonOut:
     return MortonOut(b[:3].copy(), float(b[3]), float(b[4]), lanes, codes)
 
 
-def sort(codes):
+def sort(codes, radix_bits: int = 0):
     """GPU counterpart of reference_sort (solvers/src/lbvh.rs:143-147); uint64 codes
-    select the 63-bit key sort."""
+    select the 63-bit key sort. radix_bits (8, 9 or 10; 0 = 8) sets the digit width."""
     if np.asarray(codes).dtype == np.uint64:
         c = np.ascontiguousarray(codes, dtype=np.uint64)
         order = np.zeros(c.size, np.uint32)
         sc = np.zeros(c.size, np.uint64)
-        _check(lib().glx_stage_sort63(c.size, _p(c), _p(order), _p(sc)))
+        if radix_bits:
+            _check(lib().glx_stage_sort63_bits(c.size, _p(c), _p(order), _p(sc), radix_bits))
+        else:
+            _check(lib().glx_stage_sort63(c.size, _p(c), _p(order), _p(sc)))
         return order, sc
     codes = _u32(codes)
     order = np.zeros(codes.size, np.uint32)
     sc = np.zeros(codes.size, np.uint32)
-    _check(lib().glx_stage_sort(codes.size, _p(codes), _p(order), _p(sc)))
+    if radix_bits:
+        _check(lib().glx_stage_sort_bits(codes.size, _p(codes), _p(order), _p(sc), radix_bits))
+    else:
+        _check(lib().glx_stage_sort(codes.size, _p(codes), _p(order), _p(sc)))
     return order, sc
 
 
@@ -367,10 +380,11 @@ class GpuResidentLeapfrog:
 
     def __init__(self, g: float, softening: float, theta: float, precision: int = PREC_F32,
                  morton_bits: int = 30, walk: int = WALK_REFERENCE, multipole: int = 0,
-                 leaf_size: int = 1, far_theta: float = 0.0, mono_ratio: float = 0.0):
+                 leaf_size: int = 1, far_theta: float = 0.0, mono_ratio: float = 0.0,
+                 radix_bits: int = 0, group_size: int = 0, group_lanes: int = 0):
         h = C.c_void_p()
         p = _params(g, softening, theta, precision, False, morton_bits, walk, multipole,
-                    leaf_size, far_theta, mono_ratio)
+                    leaf_size, far_theta, mono_ratio, radix_bits, group_size, group_lanes)
         _check(lib().glx_sys_create(C.byref(p), C.byref(h)))
         self._h = h
         self._mass = np.zeros(0)

@@ -89,11 +89,12 @@ class Pool:
 
     def __init__(self, budget_bytes: int, max_jobs: int = 4096, morton_bits: int = 30,
                  walk: int = WALK_REFERENCE, multipole: int = 0, precision: int = PREC_F32,
-                 leaf_size: int = 1, far_theta: float = 0.0, mono_ratio: float = 0.0):
+                 leaf_size: int = 1, far_theta: float = 0.0, mono_ratio: float = 0.0,
+                 radix_bits: int = 0, group_size: int = 0, group_lanes: int = 0):
         L = _bind()
         h = C.c_void_p()
         p = _params(1.0, 0.0, 1.0, precision, False, morton_bits, walk, multipole, leaf_size,
-                    far_theta, mono_ratio)
+                    far_theta, mono_ratio, radix_bits, group_size, group_lanes)
         _check(L.glx_pool_create(C.byref(p), int(budget_bytes), int(max_jobs), C.byref(h)))
         self._h = h
         self._n = {}

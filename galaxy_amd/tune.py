@@ -1,4 +1,6 @@
-"""Autotuner for the group walk's knobs on a given workload.
+"""Autotuner for the batched path's knobs on a given workload.
+
+Accuracy knobs of the group walk (each candidate is error-checked):
 
 * leaf size: a node of at most `leaf_size` bodies that fails the opening test is summed
   body by body instead of opened further — larger buckets trade quadrupole terms and node
@@ -10,20 +12,33 @@
   summed as a monopole alone (14 operations per body instead of 33) — the quadrupole is
   kept for the nearer accepted nodes, where it matters.
 
+Walk block shape (error-checked too: the groups' boxes decide which nodes are accepted):
+
+* group_size: bodies per walk group, one wave per group (the subtrees of at most this
+  many bodies) — smaller groups accept nodes nearer to their bodies and sum fewer terms
+  per body, but test more nodes per body and leave lanes idle.
+* group_lanes: most lanes per body in groups of at most 32 bodies (the lanes of a body
+  take every lanes-th list entry and are folded before the store).
+
+Sort radix width (8, 9 or 10 bits per LSD pass): fewer passes against more bins per
+tile. The sorted order, hence every result, does not depend on it, so it is timed only.
+
 The best combination depends on the particle distribution and theta, so the tuner times
 pool steps of a sample of the workload for each candidate and keeps the fastest whose
 force error on seeded targets (against an FP64 direct sum) stays within both bounds: the
 median within `err_bound` x `margin` (the margin absorbs the difference between the
 sample and the whole workload) and the 99th percentile within `p99_bound`. The default
 p99 bound, 1e-2, is below the reference's own per-target monopole walk at theta = 0.5
-(p99 1.4e-2 on a 16k Plummer sphere).
-
-The walk's block shape (one wave per group of at most 64 bodies, 2-4 lanes per body in
-groups of at most 32) and the sort's radix width (8 bits) are fixed by the kernels' LDS
-layout and are not tuned here.
+(p99 1.4e-2 on a 16k Plummer sphere). The search runs in stages (accuracy knobs, then
+block shape at the chosen knobs, then radix width at the chosen knobs and shape) rather
+than over the full product. `TuneCache` keeps results in a JSON file keyed by the
+workload's size bucket and theta.
 """
 from __future__ import annotations
 
+import json
+import math
+import os
 import time
 
 import numpy as np
@@ -35,13 +50,20 @@ CANDIDATES = (1, 2, 4, 8, 16)
 LEAF_SIZES = (1, 2)
 FAR_THETAS = (0.0, 0.5, 0.55, 0.6)
 MONO_RATIOS = (0.0, 0.7, 0.8, 0.9)
+GROUP_SHAPES = ((64, 4), (64, 1), (48, 4), (32, 4))
+RADIX_WIDTHS = (8, 9, 10)
 
 
-def _pool_rate(systems, leaf, mode, softening, theta, dt, steps, far=0.0, mono=0.0):
+def _pool(systems, knobs, mode, softening, theta, dt):
     need = sum(Pool.job_bytes(m.size) for _, _, m in systems)
-    pool = Pool(need + (1 << 20), max_jobs=len(systems), leaf_size=leaf, far_theta=far,
-                mono_ratio=mono, **mode)
+    pool = Pool(need + (1 << 20), max_jobs=len(systems), **mode, **knobs)
     ids = [pool.admit(p, v, m, softening=softening, theta=theta, dt=dt) for p, v, m in systems]
+    return pool, ids
+
+
+def _pool_rate(systems, leaf, mode, softening, theta, dt, steps, far=0.0, mono=0.0, **shape):
+    knobs = dict(leaf_size=leaf, far_theta=far, mono_ratio=mono, **shape)
+    pool, ids = _pool(systems, knobs, mode, softening, theta, dt)
     pool.step()  # captures the graph
     t0 = time.perf_counter()
     dev = [pool.step() for _ in range(steps)]
@@ -50,41 +72,145 @@ def _pool_rate(systems, leaf, mode, softening, theta, dt, steps, far=0.0, mono=0
     return pool, ids, n * steps / wall, float(np.median(dev))
 
 
+def _errors(pool, ids, systems, picks, softening):
+    errs = []
+    for j, (_, _, m), tg in zip(ids, systems, picks):
+        pos, _ = pool.snapshot(j)
+        acc = pool.snapshot_accel(j)
+        ref = lbvh.direct_sum_f64(pos, m, 1.0, softening, tg.astype(np.uint32))
+        errs.append(np.linalg.norm(acc[tg] - ref, axis=1) / np.linalg.norm(ref, axis=1))
+    e = np.concatenate(errs)
+    return float(np.median(e)), float(np.quantile(e, 0.99))
+
+
+def _picks(systems, per_system, seed):
+    rng = np.random.default_rng(seed)
+    return [np.sort(rng.choice(m.size, min(m.size, per_system), replace=False))
+            for _, _, m in systems]
+
+
+def _best(table, err_bound, margin, p99_bound):
+    ok = [r for r in table
+          if r["median_err"] <= err_bound * margin and r["p99_err"] <= p99_bound] or table
+    return max(ok, key=lambda r: r["rate"])
+
+
 def tune_leaf_size(systems, *, softening, theta, dt=1e-3, mode=None, steps=3,
                    err_bound=1e-3, p99_bound=1e-2, margin=0.8, per_system=32, candidates=None,
-                   seed=5, leaf_sizes=None, far_thetas=None, mono_ratios=None):
+                   seed=5, leaf_sizes=None, far_thetas=None, mono_ratios=None, shape=None):
     """Time `steps` pool steps of `systems` (list of (pos, vel, mass)) for each candidate
     (leaf size, far_theta, mono_ratio); return {"leaf_size", "far_theta", "mono_ratio",
     "table"}, where each table row has the rate (particle-forces/s), the median device step
     time and the median and p99 force errors. `candidates` alone tunes the leaf size with
-    the far field and the monopole band off."""
+    the far field and the monopole band off. `shape` (group_size, group_lanes,
+    radix_bits as keywords) is passed to every candidate's pool."""
     mode = mode or dict(morton_bits=63, walk=lbvh.WALK_GROUP, multipole=2)
+    shape = shape or {}
     if candidates is not None:
         combos = [(leaf, 0.0, 0.0) for leaf in candidates]
     else:
         combos = [(leaf, far, mono) for leaf in (leaf_sizes or LEAF_SIZES)
                   for far in (far_thetas if far_thetas is not None else FAR_THETAS)
                   for mono in (mono_ratios if mono_ratios is not None else MONO_RATIOS)]
-    rng = np.random.default_rng(seed)
-    picks = [np.sort(rng.choice(m.size, min(m.size, per_system), replace=False))
-             for _, _, m in systems]
+    picks = _picks(systems, per_system, seed)
     table = []
     for leaf, far, mono in combos:
         pool, ids, rate, ms = _pool_rate(systems, leaf, mode, softening, theta, dt, steps, far,
-                                         mono)
-        errs = []
-        for j, (_, _, m), tg in zip(ids, systems, picks):
-            pos, _ = pool.snapshot(j)
-            acc = pool.snapshot_accel(j)
-            ref = lbvh.direct_sum_f64(pos, m, 1.0, softening, tg.astype(np.uint32))
-            errs.append(np.linalg.norm(acc[tg] - ref, axis=1) / np.linalg.norm(ref, axis=1))
+                                         mono, **shape)
+        med, p99 = _errors(pool, ids, systems, picks, softening)
         pool.close()
-        e = np.concatenate(errs)
         table.append({"leaf_size": leaf, "far_theta": far, "mono_ratio": mono, "rate": rate,
-                      "device_ms": ms, "median_err": float(np.median(e)),
-                      "p99_err": float(np.quantile(e, 0.99))})
-    ok = [r for r in table
-          if r["median_err"] <= err_bound * margin and r["p99_err"] <= p99_bound] or table
-    best = max(ok, key=lambda r: r["rate"])
+                      "device_ms": ms, "median_err": med, "p99_err": p99})
+    best = _best(table, err_bound, margin, p99_bound)
     return {"leaf_size": best["leaf_size"], "far_theta": best["far_theta"],
             "mono_ratio": best["mono_ratio"], "table": table}
+
+
+def _phase_ms(systems, knobs, mode, softening, theta, dt, steps, phase):
+    """Median over `steps` timed pool steps of one phase's device time (ms)."""
+    pool, ids = _pool(systems, knobs, mode, softening, theta, dt)
+    pool.step()
+    ms = float(np.median([pool.step_timed()[phase] for _ in range(steps)]))
+    return pool, ids, ms
+
+
+def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e-3,
+         p99_bound=1e-2, margin=0.8, per_system=32, seed=5, leaf_sizes=None, far_thetas=None,
+         mono_ratios=None, group_shapes=None, radix_widths=None, radix_systems=None):
+    """All knobs, in three stages on the sample `systems`:
+    1. leaf size x far_theta x mono_ratio at the default block shape and radix width
+       (`tune_leaf_size`);
+    2. block shape (group_size, group_lanes) at the chosen accuracy knobs: each shape is
+       timed by its walk phase and error-checked;
+    3. radix width at the chosen knobs and shape, timed by the sort phase on
+       `radix_systems` (default `systems`; the whole workload where it is at hand: the
+       sort's cost per key changes with the number of tiles in flight, and a sample
+       fills the GPU less than the workload).
+    Returns {"leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes",
+    "radix_bits", "table", "shape_table", "radix_table"}."""
+    mode = mode or dict(morton_bits=63, walk=lbvh.WALK_GROUP, multipole=2)
+    r = tune_leaf_size(systems, softening=softening, theta=theta, dt=dt, mode=mode,
+                       steps=steps, err_bound=err_bound, p99_bound=p99_bound, margin=margin,
+                       per_system=per_system, seed=seed, leaf_sizes=leaf_sizes,
+                       far_thetas=far_thetas, mono_ratios=mono_ratios)
+    knobs = dict(leaf_size=r["leaf_size"], far_theta=r["far_theta"],
+                 mono_ratio=r["mono_ratio"])
+    n = sum(m.size for _, _, m in systems)
+    picks = _picks(systems, per_system, seed)
+    shape_table = []
+    for gs, lanes in (group_shapes or GROUP_SHAPES):
+        k = dict(knobs, group_size=gs, group_lanes=lanes)
+        pool, ids, ms = _phase_ms(systems, k, mode, softening, theta, dt, steps, "walk")
+        med, p99 = _errors(pool, ids, systems, picks, softening)
+        pool.close()
+        shape_table.append({"group_size": gs, "group_lanes": lanes, "walk_ms": ms,
+                            "rate": n / (ms * 1e-3), "median_err": med, "p99_err": p99})
+    shape = _best(shape_table, err_bound, margin, p99_bound)
+    knobs.update(group_size=shape["group_size"], group_lanes=shape["group_lanes"])
+    radix_table = []
+    for bits in (radix_widths or RADIX_WIDTHS):
+        pool, _, ms = _phase_ms(radix_systems or systems, dict(knobs, radix_bits=bits), mode,
+                                softening, theta, dt, steps, "sort")
+        pool.close()
+        radix_table.append({"radix_bits": bits, "sort_ms": ms})
+    radix = min(radix_table, key=lambda row: row["sort_ms"])
+    out = dict(knobs, radix_bits=radix["radix_bits"])
+    out.update(table=r["table"], shape_table=shape_table, radix_table=radix_table)
+    return out
+
+
+KNOBS = ("leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes", "radix_bits")
+
+
+class TuneCache:
+    """Tuning results in a JSON file, keyed by the workload's size bucket (log2 of the
+    total and of the mean bodies per system), theta and build. `lookup` returns the knobs
+    of a stored entry or None; `store` records `tune`'s result and rewrites the file."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self.entries = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                self.entries = json.load(f)
+
+    @staticmethod
+    def key(sizes, theta: float, mode: dict | None = None) -> str:
+        sizes = np.asarray(sizes, dtype=np.float64)
+        total, mean = float(sizes.sum()), float(sizes.mean())
+        mode = mode or {}
+        build = f"k{mode.get('morton_bits', 63)}w{mode.get('walk', lbvh.WALK_GROUP)}" \
+                f"q{mode.get('multipole', 2)}"
+        return (f"N2^{int(math.floor(math.log2(max(total, 1.0))))}"
+                f"/n2^{int(math.floor(math.log2(max(mean, 1.0))))}/theta{theta:g}/{build}")
+
+    def lookup(self, key: str):
+        e = self.entries.get(key)
+        return None if e is None else {k: e[k] for k in KNOBS}
+
+    def store(self, key: str, result: dict):
+        self.entries[key] = {k: result[k] for k in KNOBS}
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.entries, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)

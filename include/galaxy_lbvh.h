@@ -66,6 +66,12 @@ typedef struct glx_params {
                            s / (theta mono_ratio) + delta from every point of the group) is
                            summed as a monopole alone; nearer accepted nodes keep the
                            quadrupole */
+  /* Tuning (no reference counterpart; results do not depend on radix_bits): */
+  uint32_t radix_bits;  /* sort digit width, 0 = 8; 8, 9 or 10 bits per LSD pass */
+  uint32_t group_size;  /* GLX_WALK_GROUP block shape: bodies per walk group (the subtrees
+                           of at most this many bodies), 0 = 64; 8..64 */
+  uint32_t group_lanes; /* GLX_WALK_GROUP block shape: most lanes per body in groups of at
+                           most 32 bodies, 0 = 4; 1..4 */
 } glx_params;
 
 /* FP64 conservation diagnostics (core/src/diagnostics.rs:8-68). */
@@ -179,6 +185,12 @@ int glx_stage_morton63(size_t n, const double *pos, double *bounds_out, uint32_t
  * the sorted codes. Codes must be < 2^32 (30-bit Morton codes in practice). */
 int glx_stage_sort(size_t n, const uint32_t *codes, uint32_t *order, uint32_t *sorted_codes);
 int glx_stage_sort63(size_t n, const uint64_t *codes, uint32_t *order, uint64_t *sorted_codes);
+/* The same with radix_bits-wide digits (8, 9 or 10; no reference counterpart): every
+ * code must fit the passes' bits (32, 36, 30 for 32-bit codes; 64, 63, 70 for 64-bit). */
+int glx_stage_sort_bits(size_t n, const uint32_t *codes, uint32_t *order,
+                        uint32_t *sorted_codes, uint32_t radix_bits);
+int glx_stage_sort63_bits(size_t n, const uint64_t *codes, uint32_t *order,
+                          uint64_t *sorted_codes, uint32_t radix_bits);
 
 /* reference_karras (solvers/src/lbvh.rs:276-306): left/right (n-1) and parent
  * (2n-1, unified index; parent[root] = 0xFFFFFFFF). n > 0. */

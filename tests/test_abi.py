@@ -7,6 +7,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 import galaxy_amd
@@ -81,10 +82,32 @@ def test_invalid_params_rejected_before_device():
         L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, 0.0, 1.0),  # mono_ratio >= 1
         L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 63, 1, 0.0, 0.5),  # ... needs quadrupoles
         L.Params(1.0, 0.05, 0.5, 1, 2, 0, 0, 63, 1, 0.0, -0.1),
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 30, 0, 0.0, 0.0, 7),   # radix_bits below 8
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 30, 0, 0.0, 0.0, 11),  # ... above 10
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 63, 1, 0.0, 0.0, 0, 4),   # group_size below 8
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 63, 1, 0.0, 0.0, 0, 65),  # ... above 64
+        L.Params(1.0, 0.05, 0.5, 1, 0, 0, 0, 63, 1, 0.0, 0.0, 0, 0, 5),  # group_lanes above 4
     ):
         assert lib.glx_solver_create(C.byref(bad), C.byref(h)) == L.GLX_EINVAL
         assert lib.glx_last_error()
     assert lib.glx_solver_create(None, C.byref(h)) == L.GLX_EINVAL
+
+
+def test_sort_width_arguments_rejected_before_device():
+    """A digit width outside 8..10, or a code beyond the bits its passes cover (30 for
+    10-bit digits over 32-bit codes, 63 for 9-bit digits over 64-bit codes), is an
+    argument error, found before any device work."""
+    lib = L.lib()
+    o32 = np.zeros(4, np.uint32)
+    o64 = np.zeros(4, np.uint64)
+    ok32 = np.array([1, 2, 3, 4], np.uint32)
+    for bits in (7, 11):
+        assert lib.glx_stage_sort_bits(4, L._p(ok32), L._p(o32), None, bits) == L.GLX_EINVAL
+    wide32 = np.array([1, 2, 1 << 30, 4], np.uint32)
+    assert lib.glx_stage_sort_bits(4, L._p(wide32), L._p(o32), None, 10) == L.GLX_EINVAL
+    wide64 = np.array([1, 2, 1 << 63, 4], np.uint64)
+    assert lib.glx_stage_sort63_bits(4, L._p(wide64), L._p(o32), L._p(o64), 9) == L.GLX_EINVAL
+    assert "passes" in lib.glx_last_error().decode()
 
 
 def test_missing_gpu_is_an_error_not_a_fallback():

@@ -319,3 +319,61 @@ def test_far_field_collision_error():
     e = _errs(a[tg], d64)
     say(f"collision 2^18 far_theta 0.5: {e}")
     assert e[0] <= 5e-4
+
+
+# ------------------------------------------------------------------ tuning knobs
+
+
+def test_radix_width_leaves_forces_and_stepping_bit_identical():
+    """The sort's digit width changes only the pass count (and, for an odd count, the
+    buffer the sort starts in): forces and resident steps are bit-identical for 8, 9
+    and 10 bits, in the reference build (30-bit keys: 4, 4, 3 passes) and the tuned fast
+    build (63-bit keys: 8, 7, 7 passes)."""
+    pos, vel, mass = I.plummer_numpy(20000, seed=11)
+    tuned = dict(FAST, far_theta=0.55, mono_ratio=0.8)
+    for kw in ({}, tuned):
+        base = glx.Lbvh(1.0, 0.01, 0.5, **kw).accelerations(pos, mass)
+        for bits in (9, 10):
+            say("radix width", bits, kw)
+            got = glx.Lbvh(1.0, 0.01, 0.5, radix_bits=bits, **kw).accelerations(pos, mass)
+            assert np.array_equal(got, base), (bits, kw)
+    runs = []
+    for bits in (8, 9, 10):
+        sim = glx.GpuResidentLeapfrog(1.0, 0.01, 0.5, radix_bits=bits, **tuned)
+        sim.upload(pos, vel, mass)
+        sim.step_many(1e-3, 4)
+        runs.append(sim.snapshot())
+        sim.close()
+    for r in runs[1:]:
+        assert np.array_equal(r.pos, runs[0].pos) and np.array_equal(r.vel, runs[0].vel)
+
+
+def test_walk_block_shape_group_size_and_lanes():
+    """Walk block shape: group_size bounds the walk groups (subtrees of at most that many
+    bodies), group_lanes the lanes per body in groups of at most 32. Every shape is the
+    direct sum at theta -> 0. At theta = 0.5 (Plummer N=16384 against the FP64 direct sum)
+    smaller groups test more nodes and sum fewer terms per body, accepting nodes nearer to
+    their bodies: the median error grows (1.4e-4 at 64 bodies, 2.2e-4 at 16) but every
+    shape stays within 5e-4, well below the per-target reference walk's. With one group
+    size the lane count changes only the summation order."""
+    for gs in (8, 16, 32, 48):
+        s = glx.Lbvh(G, EPS, 1e-6, group_size=gs, **FAST)
+        for seed in range(3):
+            pos, _, mass = I.cluster(seed, 300)
+            assert I.rms_rel_err(s.accelerations(pos, mass), O.direct_sum(pos, mass, G, EPS)) < 2e-5
+    pos, _, mass = I.plummer_numpy(16384, seed=9)
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, 0.01, np.arange(16384, dtype=np.uint32))
+    rows = {}
+    for gs in (64, 32, 16):
+        for lanes in (1, 2, 4):
+            s = glx.Lbvh(1.0, 0.01, 0.5, count=True, group_size=gs, group_lanes=lanes, **FAST)
+            a = s.accelerations(pos, mass)
+            st = s.stats()
+            rows[gs, lanes] = (a, _errs(a, d64), st)
+            say(f"group_size {gs} lanes {lanes}: err {rows[gs, lanes][1]} stats {st}")
+            assert rows[gs, lanes][1][0] <= 5e-4
+        for lanes in (1, 2):
+            assert I.rms_rel_err(rows[gs, lanes][0], rows[gs, 4][0]) < 1e-5, (gs, lanes)
+    t64, t16 = rows[64, 4][2], rows[16, 4][2]
+    assert t16["rounds"] > t64["rounds"]
+    assert t16["point_terms"] + t16["quad_terms"] < t64["point_terms"] + t64["quad_terms"]

@@ -169,3 +169,48 @@ def test_autotuner_three_knobs_respects_both_bounds():
     base = next(t for t in r["table"] if t["far_theta"] == 0.0 and t["mono_ratio"] == 0.0)
     mono = next(t for t in r["table"] if t["far_theta"] == 0.0 and t["mono_ratio"] == 0.8)
     assert base["median_err"] <= mono["median_err"]  # the monopole band only adds error
+
+
+@pytest.mark.parametrize("mode", ["reference", "far"])
+def test_pool_radix_width_bit_identical(mode):
+    """The sort's digit width changes only the pass count: a batch of uneven systems steps
+    bit-identically with 8, 9 and 10-bit digits through the captured step graph."""
+    sy = systems([4096, 3, 2049, 7000], seed=80)
+    out = []
+    for bits in (8, 9, 10):
+        say("pool radix", mode, bits)
+        p = Pool(GB, radix_bits=bits, **MODES[mode])
+        ids = [p.admit(pos, vel, mass, softening=0.01, theta=0.5, dt=1e-3)
+               for pos, vel, mass in sy]
+        for _ in range(3):
+            p.step()
+        out.append([p.snapshot(j) for j in ids])
+        p.close()
+    for run in out[1:]:
+        for (pa, va), (pb, vb) in zip(run, out[0]):
+            assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+
+
+def test_autotuner_block_shape_and_radix_width():
+    """The staged tuner picks a block shape within both error bounds and the radix width
+    with the fastest sort phase, and its cache file returns the same knobs."""
+    from galaxy_amd.tune import TuneCache, tune
+
+    sy = systems([6000, 2500, 9000, 1200], seed=72)
+    r = tune(sy, softening=0.01, theta=0.5, steps=3, leaf_sizes=(1,), far_thetas=(0.55,),
+             mono_ratios=(0.8,), group_shapes=((64, 4), (32, 4), (32, 1)))
+    say("tuned", {k: r[k] for k in ("group_size", "group_lanes", "radix_bits")},
+        r["shape_table"], r["radix_table"])
+    assert len(r["table"]) == 1 and len(r["shape_table"]) == 3 and len(r["radix_table"]) == 3
+    ok = [t for t in r["shape_table"] if t["median_err"] <= 8e-4 and t["p99_err"] <= 1e-2]
+    best = next(t for t in r["shape_table"]
+                if (t["group_size"], t["group_lanes"]) == (r["group_size"], r["group_lanes"]))
+    assert best in ok and best["rate"] == max(t["rate"] for t in ok)
+    assert r["radix_bits"] == min(r["radix_table"], key=lambda t: t["sort_ms"])["radix_bits"]
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tune.json")
+        key = TuneCache.key([m.size for _, _, m in sy], 0.5)
+        TuneCache(path).store(key, r)
+        assert TuneCache(path).lookup(key) == {k: r[k] for k in (
+            "leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes", "radix_bits")}

@@ -113,6 +113,33 @@ def test_sort_bit_exact_million_and_ragged():
     assert_sort_exact(codes, "clustered_ties")
 
 
+@pytest.mark.parametrize("bits", [8, 9, 10])
+def test_sort_radix_widths_bit_exact(bits):
+    """Every digit width gives the reference (code, index) order: a stable LSD sort's
+    result does not depend on the width. 30-bit and 63-bit codes, ragged tiles, ties;
+    10-bit digits over 30-bit codes (3 passes) and 9- or 10-bit digits over 63-bit codes
+    (7 passes) take an odd pass count, which starts in the second buffer."""
+    rng = np.random.default_rng(40 + bits)
+    top = 1 << (30 if bits == 10 else 32)
+    for n in (1 << 20, 2048 * 3 + 17, 2047, 255, 1):
+        codes = rng.integers(0, top, n, dtype=np.uint64).astype(np.uint32)
+        say("sort bits", bits, "u32", n)
+        order, sc = glx.sort(codes, radix_bits=bits)
+        want = O.reference_sort(codes)
+        assert np.array_equal(order, want) and np.array_equal(sc, codes[want]), (bits, n)
+    codes = (rng.integers(0, 50, 1 << 18, dtype=np.uint32) << 20).astype(np.uint32)
+    order, _ = glx.sort(codes, radix_bits=bits)
+    assert np.array_equal(order, O.reference_sort(codes)), "clustered ties"
+    top63 = 1 << 63
+    for n in (1 << 20, 2048 * 5 + 3, 100):
+        codes = rng.integers(0, top63, n, dtype=np.uint64)
+        codes[: n // 4] = codes[n // 2]  # ties
+        say("sort bits", bits, "u64", n)
+        order, sc = glx.sort(codes, radix_bits=bits)
+        want = O.reference_sort63(codes)
+        assert np.array_equal(order, want) and np.array_equal(sc, codes[want]), (bits, n)
+
+
 def test_sort_is_deterministic():
     _, _, codes = O.reference_morton(I.cloud(0x50, 6000, 3.0))
     a, b = glx.sort(codes), glx.sort(codes)
