@@ -1224,6 +1224,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const float eps2 = s.eps2;
     const uint32_t bucket = s.leaf_size;
     const bool loose_theta = s.theta > 0.55f;  // the disjoint-box test is needed
+    const uint64_t leaf_base = (uint64_t)s.leaf, node_base = (uint64_t)s.node;
     // the walk groups (subtrees of at most kGroup bodies, marked by the tree build)
     // that start in this chunk of sorted leaves, one after the other
     const uint32_t ge = lane < tl.count ? gbl(s.gend)[tl.start + lane] : kEnd;
@@ -1283,20 +1284,24 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           n_vis += m;
           ++n_rounds;
         }
-        // Classify without divergent branches: every lane loads a valid leaf and node
-        // record (index 0 stands in for the other kind) and the action is selected, so
-        // the ballots below always see the full wave.
-        // act: 0 nothing, 1 point/monopole, 2 quadrupole node, 3 open, 4 bucket (a node
-        // of at most leaf_size bodies that is not accepted: its bodies are summed directly)
+        // Classify without divergent branches: every lane loads one record and the node
+        // fields (index 0 stands in for a leaf's) and the predicates below are balloted,
+        // so the ballots that build the lists always see the full wave.
+        // Lists: point/monopole, quadrupole node, open (children pushed), bucket (a node of
+        // at most leaf_size bodies that is not accepted: its bodies are summed directly),
+        // far (local expansion).
         const bool live = u != kEnd;
         const bool is_leaf = live && u < n;
         const bool is_node = live && u >= n;
-        const uint32_t li = is_leaf ? u : 0u;
         const uint32_t ni = is_node ? u - n : 0u;
-        const float4 LP = gload4(s.leaf, li);
         const size_t nr4 = 4 * (size_t)(n > 1 ? ni : 0u);  // node record as four float4
         const float4* nrec = reinterpret_cast<const float4*>(s.node);
-        const float4 A = gload4(nrec, nr4), B = gload4(nrec, nr4 + 1), Cc = gload4(nrec, nr4 + 2);
+        // a leaf's record (position, mass) has the layout of a node's first float4 (com,
+        // mass), so one load serves both kinds
+        const uint64_t abase = is_leaf ? leaf_base : node_base;
+        const uint32_t aidx = is_leaf ? u : 4u * (n > 1 ? ni : 0u);
+        const float4 A = gload4(reinterpret_cast<const float4*>(abase), aidx);
+        const float4 B = gload4(nrec, nr4 + 1), Cc = gload4(nrec, nr4 + 2);
         const float4 Df = gload4(nrec, nr4 + 3);
         const uint4 D = make_uint4(__float_as_uint(Df.x), __float_as_uint(Df.y),
                                    __float_as_uint(Df.z), __float_as_uint(Df.w));
@@ -1312,51 +1317,51 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool apart = loose_theta ? ((Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) |
                                           (B.y > gy1) | (Cc.z < gz0) | (B.z > gz1))
                                        : true;
-        const bool node_live = is_node & (A.w > 0.0f);
+        const bool massive = A.w > 0.0f;
+        const bool node_live = is_node & massive;
         const bool accept = node_live & apart & (dmin2 >= B.w);
         const uint32_t nlo = D.z, ncount = D.w - D.z + 1u;
         const bool near = node_live & !accept;
         const bool bkt = near & (ncount <= bucket);
         const bool open = near & !bkt;
-        // act 5: accepted and well separated from the whole group -> local expansion
-        // (node radius about its com from the record)
+        // accepted and well separated from the whole group -> local expansion (node radius
+        // about its com from the record)
         bool far = false;
         if (FAR) {
           const float rn = __uint_as_float(D.y) + rg;
           far = accept & (rn * rn <= s.far2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
         }
         // an accepted node that also passes the opening test at theta * mono_ratio is a
-        // monopole (act 1): the squared band radius is in the record
+        // monopole: the squared band radius is in the record
         const bool mono = QUAD && accept && dmin2 >= __uint_as_float(D.x);
-        const bool point = (is_leaf & (LP.w > 0.0f)) | (!QUAD & accept) | (mono & !far);
-        const int act = point ? 1 : (accept ? (far ? 5 : 2) : (open ? 3 : (bkt ? 4 : 0)));
-        const float4 P = is_leaf ? LP : A;
+        const bool point = (is_leaf & massive) | (!QUAD & accept) | (mono & !far);
+        const bool quadn = QUAD & accept & !point & !far;
         const uint32_t cl = left_child(__float_as_uint(Cc.w), n),
                        cr = right_child(__float_as_uint(Cc.w), n);
-        const unsigned long long m1 = __ballot(act == 1), m2 = __ballot(act == 2),
-                                 m3 = __ballot(act == 3), m4 = __ballot(act == 4);
+        const unsigned long long m1 = __ballot(point), m2 = __ballot(quadn),
+                                 m3 = __ballot(open), m4 = __ballot(bkt);
         const uint32_t c3 = (uint32_t)__popcll(m3);
         if (sp + 2u * c3 > (uint32_t)kStackCap) {
           overflow = true;
           break;
         }
-        if (act == 1) pl[np + __popcll(m1 & lt)] = P;
-        if (QUAD && act == 2) {  // only accepted nodes fetch their moments
+        if (point) pl[np + __popcll(m1 & lt)] = A;
+        if (QUAD && quadn) {  // only accepted nodes fetch their moments
           const uint32_t q = nq + __popcll(m2 & lt);
           const float4 q1 = gload4(s.quad, 2 * (size_t)ni + 1);
-          qa[q] = P;
+          qa[q] = A;
           qb[q] = gload4(s.quad, 2 * (size_t)ni);
           qc[q] = make_float2(q1.x, q1.y);
         }
-        if (act == 3) {
+        if (open) {
           const uint32_t q = sp + 2u * (uint32_t)__popcll(m3 & lt);
           stk[q] = cr;
           stk[q + 1] = cl;  // left child popped first
         }
-        if (act == 4) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
+        if (bkt) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
         if (FAR) {
-          const unsigned long long m5 = __ballot(act == 5);
-          if (act == 5) fl[nf + __popcll(m5 & lt)] = ni;
+          const unsigned long long m5 = __ballot(far);
+          if (far) fl[nf + __popcll(m5 & lt)] = ni;
           nf += (uint32_t)__popcll(m5);
         }
         np += (uint32_t)__popcll(m1);

@@ -24,10 +24,12 @@ Sort radix width (8, 9 or 10 bits per LSD pass): fewer passes against more bins 
 tile. The sorted order, hence every result, does not depend on it, so it is timed only.
 
 The best combination depends on the particle distribution and theta, so the tuner times
-pool steps of a sample of the workload for each candidate and keeps the fastest whose
-force error on seeded targets (against an FP64 direct sum) stays within both bounds: the
-median within `err_bound` x `margin` (the margin absorbs the difference between the
-sample and the whole workload) and the 99th percentile within `p99_bound`. The default
+pool steps of a sample of the workload for each candidate (device time of the step graph,
+which separates candidates a few per cent apart that wall-clock time of a few steps does
+not) and keeps the fastest whose force error on seeded targets (against an FP64 direct
+sum) stays within both bounds: the median within `err_bound` x `margin` (the margin
+absorbs the difference between the sample and the whole workload) and the 99th
+percentile within `p99_bound`. The default
 p99 bound, 1e-2, is below the reference's own per-target monopole walk at theta = 0.5
 (p99 1.4e-2 on a 16k Plummer sphere). The search runs in stages (accuracy knobs, then
 block shape at the chosen knobs, then radix width at the chosen knobs and shape) rather
@@ -39,7 +41,6 @@ from __future__ import annotations
 import json
 import math
 import os
-import time
 
 import numpy as np
 
@@ -62,14 +63,14 @@ def _pool(systems, knobs, mode, softening, theta, dt):
 
 
 def _pool_rate(systems, leaf, mode, softening, theta, dt, steps, far=0.0, mono=0.0, **shape):
+    """Pool of `systems` with the given knobs; particle-forces/s over the median device
+    time of `steps` graph-launched steps, and that time (ms)."""
     knobs = dict(leaf_size=leaf, far_theta=far, mono_ratio=mono, **shape)
     pool, ids = _pool(systems, knobs, mode, softening, theta, dt)
     pool.step()  # captures the graph
-    t0 = time.perf_counter()
-    dev = [pool.step() for _ in range(steps)]
-    wall = time.perf_counter() - t0
+    ms = float(np.median([pool.step() for _ in range(steps)]))
     n = sum(m.size for _, _, m in systems)
-    return pool, ids, n * steps / wall, float(np.median(dev))
+    return pool, ids, n / (ms * 1e-3), ms
 
 
 def _errors(pool, ids, systems, picks, softening):
@@ -90,6 +91,7 @@ def _picks(systems, per_system, seed):
 
 
 def _best(table, err_bound, margin, p99_bound):
+    """The fastest row within both error bounds (of all rows if none is)."""
     ok = [r for r in table
           if r["median_err"] <= err_bound * margin and r["p99_err"] <= p99_bound] or table
     return max(ok, key=lambda r: r["rate"])
@@ -100,8 +102,8 @@ def tune_leaf_size(systems, *, softening, theta, dt=1e-3, mode=None, steps=3,
                    seed=5, leaf_sizes=None, far_thetas=None, mono_ratios=None, shape=None):
     """Time `steps` pool steps of `systems` (list of (pos, vel, mass)) for each candidate
     (leaf size, far_theta, mono_ratio); return {"leaf_size", "far_theta", "mono_ratio",
-    "table"}, where each table row has the rate (particle-forces/s), the median device step
-    time and the median and p99 force errors. `candidates` alone tunes the leaf size with
+    "table"}, where each table row has the rate (particle-forces/s over the median device
+    step time), that time and the median and p99 force errors. `candidates` alone tunes the leaf size with
     the far field and the monopole band off. `shape` (group_size, group_lanes,
     radix_bits as keywords) is passed to every candidate's pool."""
     mode = mode or dict(morton_bits=63, walk=lbvh.WALK_GROUP, multipole=2)
