@@ -53,11 +53,11 @@ FLOPS_L2P = 60
 # by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
 # reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
 # written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 43.57e9 + 0.558e9
+WALK_TRAFFIC_BYTES = 43.562e9 + 0.558e9
 # (leaf size, far_theta, mono_ratio, group size, lanes per body) the walk counters were
 # measured on
 WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4)
-WALK_VALU_ISSUE = 0.87  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
+WALK_VALU_ISSUE = 0.89  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
@@ -569,6 +569,8 @@ def main():
     if args.profile_collision:
         fast_mode["far_theta"] = max(args.far_theta, 0.0)
         fast_mode["mono_ratio"] = max(args.mono_ratio, 0.0)
+        fast_mode.update(group_size=args.group_size, group_lanes=args.group_lanes,
+                         radix_bits=args.radix_bits)  # 0: the defaults (64, 4, 8)
         n = args.collision_n
         pos, vel, mass, _ = ic.collision(n, args.seed)
         tg = np.arange(min(n, 256), dtype=np.uint32)
