@@ -264,6 +264,10 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       }
       keys[i] = T::code(l[0], l[1], l[2]);
       vals[i] = i;
+      // a fresh build: group starts are marked by the tree build, which runs after the
+      // sort (concurrently with the gather, which therefore leaves the marks alone)
+      s.gend[i] = s.n == 1 ? 0u : kEnd;
+      if (s.n == 1) s.parent_leaf[0] = kNoParent;
     }
   }
 }
@@ -473,9 +477,8 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
 //    (solvers/src/lbvh.rs:453-456; gpu/src/fused_core.rs:32-45)
 // ----------------------------------------------------------------------------------
 
-// MARKS: a fresh build (also clears the walk-group marks); without: the positions of a
-// cached tree are refreshed in its existing leaf order.
-template <bool MARKS>
+// Sorted leaves of a fresh build, or the positions of a cached tree refreshed in its
+// existing leaf order (the same arithmetic).
 __global__ void __launch_bounds__(kBlock)
 k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -491,10 +494,6 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       lf.z = (float)(s.x[3 * i + 2] - o2);
       lf.w = (float)s.m[i];
       s.leaf[k] = lf;
-      if (MARKS) {
-        s.gend[k] = s.n == 1 ? 0u : kEnd;  // group starts are marked by the tree build
-        if (s.n == 1) s.parent_leaf[0] = kNoParent;
-      }
     }
   }
 }
@@ -1700,13 +1699,13 @@ hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint
 }
 
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_gather<true>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
   return hipGetLastError();
 }
 
 hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
                                  uint32_t net) {
-  hipLaunchKernelGGL(k_gather<false>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
   return hipGetLastError();
 }
 

@@ -48,7 +48,7 @@ class Segment {
 // by one launch sequence.
 class Batch {
  public:
-  explicit Batch(hipStream_t st) : stream_(st) {}
+  explicit Batch(hipStream_t st);
   ~Batch();
   Batch(const Batch&) = delete;
   Batch& operator=(const Batch&) = delete;
@@ -78,7 +78,8 @@ class Batch {
   void karras();
   void aggregate();
   void walk(int mode, bool count);
-  // bbox-independent tail: encode .. walk (bbox must already be accumulated)
+  // bbox-independent tail: encode .. walk (bbox must already be accumulated); the gather
+  // runs on a side stream beside the tree build (both need only the sort)
   void build_and_walk(int mode, bool count);
   void diag(double* host_out /* nseg * kDiagRecord */);
 
@@ -86,6 +87,8 @@ class Batch {
 
  private:
   hipStream_t stream_;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
   Mode mode_{};
   uint32_t nseg_ = 0;
   std::vector<SegDesc> host_;

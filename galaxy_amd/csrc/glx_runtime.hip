@@ -155,7 +155,22 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
 // Batch
 // ----------------------------------------------------------------------------------
 
+// The side stream and its events exist from construction, so that a step graph captured
+// on the batch stream never creates them mid-capture.
+Batch::Batch(hipStream_t st) : stream_(st) {
+  check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
+  if (hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess) {
+    if (fork_) (void)hipEventDestroy(fork_);
+    (void)hipStreamDestroy(side_);
+    fail(GLX_EDEVICE, "hipEventCreate failed");
+  }
+}
+
 Batch::~Batch() {
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
+  if (side_) (void)hipStreamDestroy(side_);
   (void)hipFree(segs_);
   (void)hipFree(etiles_);
   (void)hipFree(atiles_);
@@ -286,8 +301,14 @@ void Batch::walk(int mode, bool count) {
 void Batch::build_and_walk(int mode, bool count) {
   encode();
   sort();
-  gather();
+  // the gather (HBM-bound) beside the tree build (latency-bound): both read only the
+  // sort's output and write disjoint arrays; the bottom-up pass needs both
+  check(hipEventRecord(fork_, stream_), "hipEventRecord");
+  check(hipStreamWaitEvent(side_, fork_, 0), "hipStreamWaitEvent");
+  check(launch_gather(side_, segs_, etiles_, n_et_), "gather");
+  check(hipEventRecord(join_, side_), "hipEventRecord");
   karras();
+  check(hipStreamWaitEvent(stream_, join_, 0), "hipStreamWaitEvent");
   aggregate();
   walk(mode, count);
 }
