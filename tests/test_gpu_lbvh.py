@@ -73,6 +73,22 @@ def test_bit_deterministic_same_device():
     assert np.array_equal(s.accelerations(pos, mass), s.accelerations(pos, mass))
 
 
+def test_bit_deterministic_multi_tile_trees():
+    """Forces repeat bit for bit on systems whose trees span many bottom-up tiles (the
+    order in which a node's children finish varies between runs), in the reference and
+    the fast build."""
+    from galaxy_amd import ic
+
+    pos, _, mass = ic.plummer(20000, 7)
+    fast = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2, far_theta=0.55,
+                mono_ratio=0.8)
+    for kw in ({}, fast):
+        s = glx.Lbvh(1.0, 0.01, 0.5, **kw)
+        a = s.accelerations(pos, mass)
+        for _ in range(3):
+            assert np.array_equal(s.accelerations(pos, mass), a), kw
+
+
 def test_momentum_flux_at_theta_zero():
     """gpu/tests/gpu_lbvh.rs:191-207"""
     s = glx.Lbvh(G, EPS, 1e-6)

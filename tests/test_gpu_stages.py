@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import galaxy_amd as glx
+from galaxy_amd import ic
 from oracle import oracle as O
 from tests import inputs as I
 
@@ -231,6 +232,22 @@ def test_aggregate_reference_cases():
     assert g["mass"][0] == np.float32(1.25)
     assert np.array_equal(g["aabb_min"][0] + g["origin"].astype(np.float32),
                           np.array([2.0, -1.0, 0.5], np.float32))
+
+
+def test_aggregate_deterministic_on_a_deep_cross_tile_tree():
+    """A clustered input (2^18 two-galaxy collision, 30-bit keys) gives the top bottom-up
+    pass a deep tree of nodes that straddle tiles, where which child arrives last varies
+    from run to run: every node's aggregate must not depend on it (bit-identical over
+    repeated builds, with and without quadrupoles)."""
+    pos, _, mass, _ = ic.collision(1 << 18, 3)
+    sc, sp, sm = _sorted_inputs(pos, mass)
+    for quad in (False, True):
+        say("aggregate determinism", "collision_256k", quad)
+        a = glx.aggregate(sc, sp, sm, quad=quad)
+        for _ in range(3):
+            b = glx.aggregate(sc, sp, sm, quad=quad)
+            for k in ("aabb_min", "aabb_max", "com", "mass") + (("quad",) if quad else ()):
+                assert np.array_equal(a[k], b[k]), f"{k} not deterministic (quad={quad})"
 
 
 def test_aggregate_large_and_deterministic():
