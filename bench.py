@@ -3,7 +3,7 @@
 
 Headline workload (the mixed-size ensemble the served-vs-plain metric is quoted on):
 512 independent Plummer spheres (G = M = a = 1, truncated at r = 10), sizes
-N_s = floor(exp(U(ln 1024, ln 65536))) from a seed (about 7.9 M particles in all),
+N_s = floor(exp(U(ln 1024, ln 65536))) from a seed (7,482,934 particles in all),
 eps = 0.01, theta = 0.5, dt = 1e-3. Every system is resident in one pool and all are
 advanced together by one hipGraph launch per KDK step (segment-aware kernels:
 kick-drift + per-system bbox, 63-bit Morton encode, segmented radix sort, gather,
@@ -13,6 +13,10 @@ Karras build, bottom-up monopole + quadrupole aggregate, group walk + closing ki
   plain            the same systems with the same build, one system at a time through
                    the single-system resident path (glx_sys_*)
   p50/p99 step     per-step device time of the pool step (HIP events on its stream)
+  steps/s          pool steps per second, against the step's roofline (algorithmic HBM
+                   bytes of every phase at 8 TB/s + the walk's executed flops at the
+                   FP32 peak); particle-forces/s is N times steps/s, so the fraction is
+                   the same for both
   force error      |a - a_fp64| / |a_fp64| on seeded targets of every system, against an
                    FP64 direct sum at the same positions
 
@@ -21,9 +25,12 @@ through the plain single-system path, in the fast build and in the reference bui
 (30-bit keys, per-target walk, monopole: codes, order and topology bit-exact with the
 reference CPU path).
 
-Roofline: the walk (k_walk_group) dominates; it is FP32-VALU bound and counted at
-20 flops per point-mass term and 56 per quadrupole term (terms counted by the kernel on
-the same positions) over its average device time from HIP events on the pool's stream.
+Roofline: the walk (k_walk_group) dominates; it is FP32-VALU bound. `achieved` counts 20
+flops per interaction of the reference algorithm on the same positions (counted by the
+reference build); the executed figure counts 20 flops per point-mass term, 56 per
+quadrupole term, 150 per far node and 60 per body of a local expansion (terms counted by
+the kernel); both over the walk's average device time from HIP events on the pool's
+stream.
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -49,10 +56,11 @@ FLOPS_QUAD = 56
 FLOPS_M2L = 150
 FLOPS_L2P = 60
 # HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
-# `bench.py --profile --leaf-size 1 --far-theta 0.5` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled
-# by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE
-# reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 51.06 GB fetched + 0.56 GB
-# written (leaf size 1, far-field ratio 0.5); the tested node records come from beyond L2.
+# `bench.py --profile --leaf-size 1 --far-theta 0.55 --mono-ratio 0.8 --group-size 64
+# --group-lanes 4 --radix-bits 8` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled by the same
+# counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE reads half
+# the copied bytes (x2), WRITE_SIZE all of them (x1). 43.56 GB fetched + 0.56 GB written;
+# the tested node records come from beyond L2.
 WALK_TRAFFIC_BYTES = 43.562e9 + 0.558e9
 # (leaf size, far_theta, mono_ratio, group size, lanes per body) the walk counters were
 # measured on
@@ -65,8 +73,24 @@ PHASE_BYTES_63 = {"kick_drift_bbox": 72, "encode": 36, "sort": 196, "gather": 36
                   "aggregate": 264}
 
 
+HBM_PEAK_GBS = 8000.0
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def step_roofline(bytes_tab, n, walk_flops, ms_per_step):
+    """Steps/s against the step's roofline: every HBM-bound phase moving its algorithmic
+    bytes at the HBM peak plus the walk's executed flops at the FP32 peak (the executed
+    terms, not the reference algorithm's interactions, which on clustered inputs would put
+    the roofline below the measured rate)."""
+    hbm_bytes = float(sum(bytes_tab.values())) * n
+    t = hbm_bytes / (HBM_PEAK_GBS * 1e9) + walk_flops / (FP32_PEAK_TFLOPS * 1e12)
+    sps = 1e3 / ms_per_step
+    return {"steps_per_s": sps, "roofline_steps_per_s": 1.0 / t, "frac": sps * t,
+            "hbm_bytes": hbm_bytes, "walk_flops_executed": walk_flops,
+            "roofline_ms": t * 1e3}
 
 
 def ensemble_sizes(seed: int, count: int = 512, lo: int = 1024, hi: int = 65536):
@@ -225,6 +249,7 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
     return {
         "value": n_tot * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
+        "steps": step_roofline(PHASE_BYTES_63, n_tot, flops, wall * 1e3 / steps),
         "device_ms": {"p50": float(np.percentile(dev, 50)), "p99": float(np.percentile(dev, 99)),
                       "mean": float(dev.mean())},
         "phase_ms_per_step": phases,
@@ -291,6 +316,7 @@ def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets
         "value": n * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
         "graph_ms_per_step": graph_ms,
+        "steps": step_roofline(bytes_tab, n, flops, graph_ms),
         "phase_ms_per_step": {k: v / steps for k, v in phases.items()},
         "phase_hbm_gbs": {k: bytes_tab[k] * n / (phases[k] / steps * 1e-3) / 1e9
                           for k in bytes_tab},
@@ -632,6 +658,9 @@ def main():
                    "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
                    "leaf_size": leaf, "far_theta": far, "mono_ratio": mono,
                    "group_size": gsize, "group_lanes": glanes, "radix_bits": rbits},
+        "steps_per_s": b["steps"]["steps_per_s"],
+        # steps/s (and so particle-forces/s) against the whole step's roofline
+        "step_roofline": b["steps"],
         "p50_step_ms": b["device_ms"]["p50"],
         "p99_step_ms": b["device_ms"]["p99"],
         "force_error_vs_fp64_direct": b["error"],
