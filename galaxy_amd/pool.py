@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from .lbvh import (GLX_EBUDGET, GLX_ENOSLOT, GLX_OK, PREC_F32, WALK_REFERENCE, GlxError, Stats,
+from .lbvh import (GLX_EBUDGET, GLX_ENOSLOT, PREC_F32, WALK_REFERENCE, Stats,
                    _check, _f64, _p, _params, lib)
 
 
