@@ -9,6 +9,7 @@ from .lbvh import (  # noqa: F401
     Lbvh,
     State,
     aggregate,
+    flatten,
     device_info,
     direct_sum_f64,
     karras,

@@ -505,6 +505,52 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
   });
 }
 
+extern "C" int glx_stage_flatten(size_t n, const uint32_t* sorted_codes, const double* sorted_pos,
+                                 const double* sorted_mass, double* origin_out, uint32_t* next,
+                                 uint32_t* body_start, uint32_t* body_count, float* center,
+                                 float* half, float* com, float* mass, float* delta) {
+  return guard([&] {
+    validate_n(n);
+    if (n == 0) fail(GLX_EINVAL, "reference_flatten requires a non-empty system");
+    if (!sorted_codes || !sorted_pos || !sorted_mass || !next || !body_start || !body_count ||
+        !center || !half || !com || !mass || !delta)
+      fail(GLX_EINVAL, "NULL array");
+    for (size_t k = 1; k < n; ++k)
+      if (sorted_codes[k - 1] > sorted_codes[k]) fail(GLX_EINVAL, "codes must be sorted");
+    ensure_device();
+    StageCtx c(false, false);
+    c.describe((uint32_t)n);
+    h2d(c.seg.desc.x, sorted_pos, 3 * n * sizeof(double), c.S.st);
+    h2d(c.seg.desc.m, sorted_mass, n * sizeof(double), c.S.st);
+    h2d(c.seg.desc.keys[0], sorted_codes, n * sizeof(uint32_t), c.S.st);
+    c.batch.reset_bbox();
+    c.batch.bbox();
+    c.batch.bounds_only();
+    c.batch.iota();
+    c.batch.gather();
+    c.batch.karras();
+    c.batch.aggregate();
+    const size_t t = 2 * n - 1;
+    DevArray<uint32_t> u32(3 * t);
+    DevArray<float> f32(11 * t);
+    const FlatOut o{u32.p, u32.p + t, u32.p + 2 * t, f32.p, f32.p + 3 * t,
+                    f32.p + 6 * t, f32.p + 9 * t, f32.p + 10 * t};
+    c.batch.flatten(o);
+    d2h(next, o.next, t * sizeof(uint32_t), c.S.st);
+    d2h(body_start, o.body_start, t * sizeof(uint32_t), c.S.st);
+    d2h(body_count, o.body_count, t * sizeof(uint32_t), c.S.st);
+    d2h(center, o.center, 3 * t * sizeof(float), c.S.st);
+    d2h(half, o.half, 3 * t * sizeof(float), c.S.st);
+    d2h(com, o.com, 3 * t * sizeof(float), c.S.st);
+    d2h(mass, o.mass, t * sizeof(float), c.S.st);
+    d2h(delta, o.delta, t * sizeof(float), c.S.st);
+    double b[8];
+    d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
+    sync(c.S.st);
+    if (origin_out) memcpy(origin_out, b + 5, 3 * sizeof(double));
+  });
+}
+
 // ----------------------------------------------------------------------------------
 // resident stepper
 // ----------------------------------------------------------------------------------

@@ -6,6 +6,12 @@ namespace glx {
 
 constexpr int kDiagRecord = 11;  // KE, P xyz, L xyz, sum m*x xyz, sum m
 
+// Output of the pre-order flatten (2n-1 slots; float arrays of 3 hold xyz per slot).
+struct FlatOut {
+  uint32_t *next, *body_start, *body_count;
+  float *center, *half, *com, *mass, *delta;
+};
+
 hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg);
 hipError_t launch_bbox(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
 hipError_t launch_kick_drift_bbox(hipStream_t st, const SegDesc* segs, const Tile* et,
@@ -20,6 +26,8 @@ hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
 hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
                                  uint32_t net);
+hipError_t launch_flatten(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net,
+                          const FlatOut& out);
 hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net);
 hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,

@@ -884,6 +884,82 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
 }
 
 // ----------------------------------------------------------------------------------
+// 6b. flatten, exported for parity only (the walks do not need it): the reference's DFS
+//     pre-order layout (solvers/src/lbvh.rs:545-614), every node placed in parallel. In
+//     pre-order with the left child first, the nodes before u are its ancestors and the
+//     subtrees hanging left of its root path; those cover the leaves [0, lo) and hold
+//     2 lo minus their count of nodes, one per right edge of the path, so
+//     slot(u) = 2 lo + (left edges on the root path of u).
+// ----------------------------------------------------------------------------------
+
+__device__ uint32_t flat_slot(const SegDesc& s, uint32_t u) {
+  const uint32_t n = s.n;
+  if (n == 1) return 0u;
+  const uint32_t lo = u < n ? u : gbl(s.lo)[u - n];
+  uint32_t left_edges = 0, v = u;
+  uint32_t p = u < n ? gbl(s.parent_leaf)[u] : gbl(s.parent_int)[u - n];
+  for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {  // depth < n
+    const uint32_t j = p - n;
+    if (j >= n - 1) break;
+    if (gbl(s.left)[j] == v) ++left_edges;
+    v = p;
+    p = gbl(s.parent_int)[j];
+  }
+  return 2u * lo + left_edges;
+}
+
+__device__ __forceinline__ void flat_store3(float* a, uint32_t q, float x, float y, float z) {
+  a[3 * (size_t)q] = x;
+  a[3 * (size_t)q + 1] = y;
+  a[3 * (size_t)q + 2] = z;
+}
+
+// One thread per sorted leaf k places leaf k and internal node k (k < n - 1). Geometry is
+// the f32 origin-relative aggregate: centre and half extent of the AABB, com, mass and
+// delta = |com - centre|; a leaf's centre is its position and its half extent zero.
+__global__ void __launch_bounds__(kBlock)
+k_flatten(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
+          FlatOut o) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const SegDesc& s = segs[tl.seg];
+    const uint32_t n = s.n;
+    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
+      const uint32_t k = tl.start + e;
+      {
+        const uint32_t q = flat_slot(s, k);
+        const float4 lf = s.leaf[k];
+        o.next[q] = q + 1u;
+        o.body_start[q] = k;
+        o.body_count[q] = 1u;
+        flat_store3(o.center, q, lf.x, lf.y, lf.z);
+        flat_store3(o.half, q, 0.f, 0.f, 0.f);
+        flat_store3(o.com, q, lf.x, lf.y, lf.z);
+        o.mass[q] = lf.w;
+        o.delta[q] = 0.f;
+      }
+      if (k + 1u < n) {
+        const uint32_t q = flat_slot(s, n + k);
+        const Node nd = s.node[k];
+        const uint32_t lo = s.lo[k], hi = s.hi[k];
+        const float cx = (nd.b.x + nd.c.x) * 0.5f, cy = (nd.b.y + nd.c.y) * 0.5f,
+                    cz = (nd.b.z + nd.c.z) * 0.5f;
+        const float dx = nd.a.x - cx, dy = nd.a.y - cy, dz = nd.a.z - cz;
+        o.next[q] = q + 2u * (hi - lo + 1u) - 1u;  // 2 leaves - 1 nodes in the subtree
+        o.body_start[q] = 0u;
+        o.body_count[q] = 0u;
+        flat_store3(o.center, q, cx, cy, cz);
+        flat_store3(o.half, q, (nd.c.x - nd.b.x) * 0.5f, (nd.c.y - nd.b.y) * 0.5f,
+                    (nd.c.z - nd.b.z) * 0.5f);
+        flat_store3(o.com, q, nd.a.x, nd.a.y, nd.a.z);
+        o.mass[q] = nd.a.w;
+        o.delta[q] = sqrtf(dx * dx + dy * dy + dz * dz);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------
 // 7. walks. Accelerations are accumulated in origin-relative f32 coordinates with
 //    optional Kahan compensation; the epilogue scatters a (x G) to the original
 //    index and, in resident mode, applies the closing half-kick
@@ -1706,6 +1782,12 @@ hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, ui
 hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
                                  uint32_t net) {
   hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  return hipGetLastError();
+}
+
+hipError_t launch_flatten(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net,
+                          const FlatOut& out) {
+  hipLaunchKernelGGL(k_flatten, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net, out);
   return hipGetLastError();
 }
 

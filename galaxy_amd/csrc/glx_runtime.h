@@ -77,6 +77,7 @@ class Batch {
   void refresh_leaves();  // new positions into the cached leaf order (no rebuild)
   void karras();
   void aggregate();
+  void flatten(const FlatOut& out);  // pre-order export of the built tree (parity only)
   void walk(int mode, bool count);
   // bbox-independent tail: encode .. walk (bbox must already be accumulated); the gather
   // runs on a side stream beside the tree build (both need only the sort)

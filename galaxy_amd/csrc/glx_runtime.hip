@@ -294,6 +294,9 @@ void Batch::karras() { check(launch_karras(stream_, mode_, segs_, etiles_, n_et_
 void Batch::aggregate() {
   check(launch_aggregate(stream_, mode_, segs_, nseg_, atiles_, n_at_), "aggregate");
 }
+void Batch::flatten(const FlatOut& out) {
+  check(launch_flatten(stream_, segs_, etiles_, n_et_, out), "flatten");
+}
 void Batch::walk(int mode, bool count) {
   check(launch_walk(stream_, mode_, segs_, wtiles_, n_wt_, gtiles_, n_gt_, mode, count), "walk");
 }

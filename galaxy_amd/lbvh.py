@@ -118,6 +118,7 @@ def lib():
         "glx_stage_karras": (I, [S, P, P, P, P]),
         "glx_stage_karras63": (I, [S, P, P, P, P]),
         "glx_stage_aggregate": (I, [S, P, P, P, P, P, P, P, P, P]),
+        "glx_stage_flatten": (I, [S, P, P, P, P, P, P, P, P, P, P, P, P]),
         "glx_sys_create": (I, [C.POINTER(Params), C.POINTER(P)]),
         "glx_sys_destroy": (I, [P]),
         "glx_sys_upload": (I, [P, S, P, P, P]),
@@ -368,6 +369,28 @@ def aggregate(sorted_codes, sorted_pos, sorted_mass, quad: bool = False):
     out["origin"] = origin
     if quad:
         out["quad"] = q
+    return out
+
+
+def flatten(sorted_codes, sorted_pos, sorted_mass):
+    """GPU counterpart of reference_flatten (solvers/src/lbvh.rs:545-614) over the tree of
+    the sorted inputs: per pre-order slot next / body_start / body_count and f32 geometry
+    (center, half, com relative to `origin`; mass; delta). leaf_bodies = order[body_start]
+    at the leaf slots."""
+    sc = _u32(sorted_codes)
+    sp = _f64(sorted_pos, (-1, 3))
+    sm = _f64(sorted_mass)
+    n = sc.size
+    t = 2 * n - 1
+    origin = np.zeros(3)
+    out = {k: np.zeros(t, np.uint32) for k in ("next", "body_start", "body_count")}
+    out.update({k: np.zeros((t, 3), np.float32) for k in ("center", "half", "com")})
+    out.update({k: np.zeros(t, np.float32) for k in ("mass", "delta")})
+    _check(lib().glx_stage_flatten(n, _p(sc), _p(sp), _p(sm), _p(origin), _p(out["next"]),
+                                   _p(out["body_start"]), _p(out["body_count"]),
+                                   _p(out["center"]), _p(out["half"]), _p(out["com"]),
+                                   _p(out["mass"]), _p(out["delta"])))
+    out["origin"] = origin
     return out
 
 

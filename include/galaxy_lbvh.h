@@ -199,6 +199,17 @@ int glx_stage_karras(size_t n, const uint32_t *sorted_codes, uint32_t *left, uin
 int glx_stage_karras63(size_t n, const uint64_t *sorted_codes, uint32_t *left, uint32_t *right,
                        uint32_t *parent);
 
+/* reference_flatten (solvers/src/lbvh.rs:545-614; the walks here do not need it): the
+ * tree of the sorted inputs laid out in DFS pre-order, left child first, 2n-1 slots:
+ * next (the slot after the node's subtree), body_start / body_count (a leaf: its sorted
+ * index and 1, so leaf_bodies[k] = order[k]; an internal node: 0 and 0), and f32
+ * geometry relative to origin_out: center and half (3 per slot) of the AABB, com (3),
+ * mass and delta = |com - center|. The structure is the reference's exactly. n > 0. */
+int glx_stage_flatten(size_t n, const uint32_t *sorted_codes, const double *sorted_pos,
+                      const double *sorted_mass, double *origin_out, uint32_t *next,
+                      uint32_t *body_start, uint32_t *body_count, float *center, float *half,
+                      float *com, float *mass, float *delta);
+
 /* reference_aggregate (solvers/src/lbvh.rs:329-379) as computed by the device: per
  * unified node (2n-1) AABB min/max, centre of mass (3 floats each) and mass, in f32
  * coordinates relative to origin_out (the root cube centre of the sorted positions,

@@ -234,6 +234,66 @@ def test_aggregate_reference_cases():
                           np.array([2.0, -1.0, 0.5], np.float32))
 
 
+# ------------------------------------------------------------------------- flatten
+
+
+def assert_flatten(sc, sp, sm, order, label):
+    """Pre-order layout against the oracle's reference_flatten over the same tree (the
+    oracle's aggregate taken on the f32 origin-relative leaves, as in assert_aggregate):
+    next / body_start / body_count and leaf_bodies exact; geometry to f32 rounding."""
+    say("flatten", label, sc.size)
+    g = glx.flatten(sc, sp, sm)
+    n = sc.size
+    if n == 1:
+        assert list(g["next"]) == [1] and list(g["body_start"]) == [0]
+        assert list(g["body_count"]) == [1]
+        return
+    l, rr, _ = O.reference_karras(sc)
+    leaves = (sp - g["origin"]).astype(np.float32).astype(np.float64)
+    agg = O.reference_aggregate(l, rr, leaves, sm.astype(np.float32).astype(np.float64))
+    nodes, lb = O.reference_flatten(l, rr, agg, order)
+    for k in ("next", "body_start", "body_count"):
+        assert np.array_equal(g[k], nodes[k]), f"{label}: {k}"
+    leaf_slots = g["body_count"] == 1
+    assert np.array_equal(order[g["body_start"][leaf_slots]], lb), f"{label}: leaf_bodies"
+    ext = float(np.abs(leaves).max()) + 1e-30
+    for k in ("center", "half"):
+        err = np.abs(g[k].astype(np.float64) - nodes[k]).max()
+        assert err <= 2e-7 * ext, (label, k, err)
+    com_err = np.abs(g["com"].astype(np.float64) - nodes["com"]).max()
+    assert com_err <= 2e-5 * ext + 1e-6, (label, com_err)
+    d_err = np.abs(g["delta"].astype(np.float64) - nodes["delta"]).max()
+    assert d_err <= 4e-5 * ext + 2e-6, (label, d_err)
+    m_err = (np.abs(g["mass"] - nodes["mass"]) / np.maximum(nodes["mass"], 1.0)).max()
+    assert m_err <= 2e-6, (label, m_err)
+
+
+def _flatten_case(pos, mass, label):
+    _, _, codes = O.reference_morton(pos)
+    order = O.reference_sort(codes)
+    assert_flatten(codes[order], pos[order], mass[order], order.astype(np.uint32), label)
+
+
+def test_flatten_matches_reference_layout():
+    """reference_flatten (solvers/src/lbvh.rs:545-614), which the walks here do not need,
+    exported for parity: clouds, coincident bodies (index tie-breaks), the N = 1 and N = 2
+    known answers of gpu/tests/lbvh_flatten.rs:355-388, a 64k Plummer sphere and a 2^18
+    collision whose tree is deep."""
+    for seed in (3, 17):
+        _flatten_case(I.cloud(seed, 4000, 3.0), I.masses(seed ^ 0xABCD, 4000), f"cloud{seed}")
+    _flatten_case(np.tile(np.array([[1.5, -2.0, 0.25]]), (512, 1)), I.masses(99, 512),
+                  "coincident")
+    _flatten_case(np.array([[2.0, -1.0, 0.5]]), np.array([1.25]), "single")
+    g = glx.flatten(np.array([1, 6], np.uint32), np.array([[-1.0, 0, 0], [1.0, 0, 0]]),
+                    np.array([1.0, 2.0]))
+    assert list(g["next"]) == [3, 2, 3] and list(g["body_count"]) == [0, 1, 1]
+    assert list(g["body_start"]) == [0, 0, 1]
+    pos, _, mass = I.plummer_numpy(65536, seed=21)
+    _flatten_case(pos, mass, "plummer64k")
+    pos, _, mass, _ = ic.collision(1 << 18, 5)
+    _flatten_case(pos, mass, "collision256k")
+
+
 def test_aggregate_deterministic_on_a_deep_cross_tile_tree():
     """A clustered input (2^18 two-galaxy collision, 30-bit keys) gives the top bottom-up
     pass a deep tree of nodes that straddle tiles, where which child arrives last varies
