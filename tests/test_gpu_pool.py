@@ -221,3 +221,26 @@ def test_autotuner_block_shape_and_radix_width():
         TuneCache(path).store(key, r)
         assert TuneCache(path).lookup(key) == {k: r[k] for k in (
             "leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes", "radix_bits")}
+
+
+@pytest.mark.parametrize("mode", ["reference", "far"])
+def test_graph_branches_equal_the_sequential_step(mode):
+    """The captured step runs the gather on a side branch beside the Karras build; the
+    timed step runs every phase in sequence on one stream. Both give the same trajectory,
+    replay after replay, with eager work (a counting force pass, snapshots) between the
+    replays."""
+    sy = systems([5000, 1200, 3000], seed=90)
+    a, b = Pool(GB, **MODES[mode]), Pool(GB, **MODES[mode])
+    ia = [a.admit(pos, vel, mass, softening=0.01, theta=0.5, dt=1e-3) for pos, vel, mass in sy]
+    ib = [b.admit(pos, vel, mass, softening=0.01, theta=0.5, dt=1e-3) for pos, vel, mass in sy]
+    for q in range(5):
+        say("branches vs sequential", mode, q)
+        a.step()
+        b.step_timed()
+        if q == 2:
+            assert a.count() == b.count()
+        for ja, jb in zip(ia, ib):
+            (pa, va), (pb, vb) = a.snapshot(ja), b.snapshot(jb)
+            assert np.array_equal(pa, pb) and np.array_equal(va, vb), (mode, q)
+    a.close()
+    b.close()
