@@ -20,8 +20,8 @@ extern "C" int glx_device_info(char* name, size_t name_len, int* compute_units,
     ensure_device();
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
-    if (name && name_len) {
-      strncpy(name, prop.name, name_len - 1);
+    if (name && name_len) {  // the marketing name, else the gfx target (e.g. gfx950:...)
+      strncpy(name, prop.name[0] ? prop.name : prop.gcnArchName, name_len - 1);
       name[name_len - 1] = 0;
     }
     if (compute_units) *compute_units = prop.multiProcessorCount;

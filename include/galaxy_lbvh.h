@@ -100,7 +100,8 @@ typedef struct glx_stats {
 /* Message of the last failed call on this thread ("" if none). */
 const char *glx_last_error(void);
 
-/* Device name, compute-unit count and HBM bytes of device 0 (GLX_EDEVICE if none). */
+/* Device name (the gfx target where the driver reports no name), compute-unit count and
+ * HBM bytes of device 0 (GLX_EDEVICE if none). */
 int glx_device_info(char *name, size_t name_len, int *compute_units, uint64_t *hbm_bytes);
 
 /* Device memory for callers without an allocator of their own (the Python front end uses
