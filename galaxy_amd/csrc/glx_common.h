@@ -21,8 +21,8 @@ constexpr uint32_t kEnd = 0xFFFFFFFFu;       // walk terminator
 constexpr int kRadixBits = 8;                          // default sort digit width
 constexpr int kRadixBitsMax = 10;                      // widest digit (8, 9 or 10 bits)
 constexpr int kRadixBinsMax = 1 << kRadixBitsMax;
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
+constexpr int kSortThreads = 512;
+constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per element tile
 constexpr int kBlock = 256;                            // threads per block of element kernels
 constexpr int kAggTile = 1024;                         // leaves per bottom-up tile (LDS pass)

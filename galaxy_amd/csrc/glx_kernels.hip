@@ -382,7 +382,7 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
   constexpr int BINS = 1 << B;
   constexpr int W = kSortThreads / 64;
   constexpr int PER_WAVE = kSortTile / W;
-  constexpr int R = BINS / kSortThreads;  // digits per thread in the tile scan
+  constexpr int R = (BINS + kSortThreads - 1) / kSortThreads;  // digits per thread, tile scan
   __shared__ uint32_t wcnt[W][BINS];
   __shared__ uint32_t dstart[BINS];
   __shared__ K skey[kSortTile];
@@ -435,7 +435,7 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
       for (int r = 0; r < R; ++r) {
         const int d = threadIdx.x * R + r;
         uint32_t run = 0;
-        for (int k = 0; k < W; ++k) {
+        for (int k = 0; k < W && d < BINS; ++k) {
           const uint32_t c = wcnt[k][d];
           wcnt[k][d] = run;
           run += c;
@@ -446,7 +446,7 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
       uint32_t first = block_exclusive_scan(sum, shw);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        dstart[threadIdx.x * R + r] = first;
+        if (threadIdx.x * R + r < BINS) dstart[threadIdx.x * R + r] = first;
         first += tot[r];
       }
     }
