@@ -59,13 +59,13 @@ FLOPS_L2P = 60
 # `bench.py --profile --leaf-size 1 --far-theta 0.55 --mono-ratio 0.8 --group-size 64
 # --group-lanes 4 --radix-bits 8` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled by the same
 # counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE reads half
-# the copied bytes (x2), WRITE_SIZE all of them (x1). 43.62 GB fetched + 0.56 GB written;
+# the copied bytes (x2), WRITE_SIZE all of them (x1). 43.59 GB fetched + 0.56 GB written;
 # the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 43.617e9 + 0.558e9
+WALK_TRAFFIC_BYTES = 43.594e9 + 0.558e9
 # (leaf size, far_theta, mono_ratio, group size, lanes per body) the walk counters were
 # measured on
 WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4)
-WALK_VALU_ISSUE = 0.87  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
+WALK_VALU_ISSUE = 0.88  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
                   "aggregate": 168}
