@@ -528,8 +528,12 @@ __device__ __forceinline__ int kdelta(const KeyWindow<K>& kw, int n, int a, int 
   return KeyTraits<K>::clz(ca ^ cb);
 }
 
+// 1024 threads per 2048-key tile: two nodes per thread, so the binary searches of a tile
+// are spread over 16 waves that hide each other's LDS and memory latency.
+constexpr int kKarrasThreads = 1024;
+
 template <class K>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kKarrasThreads)
 k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
   __shared__ K win[kSortTile + 2 * kKarrasPad];
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -538,12 +542,12 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
     const int n = (int)s.n;
     const K* codes = static_cast<const K*>(s.keys[0]);
     KeyWindow<K> kw{codes, win, (int)tl.start - kKarrasPad, (int)tl.count + 2 * kKarrasPad};
-    for (int q = threadIdx.x; q < kw.wlen; q += kBlock) {
+    for (int q = threadIdx.x; q < kw.wlen; q += kKarrasThreads) {
       const int g = kw.w0 + q;
       if (g >= 0 && g < n) win[q] = codes[g];
     }
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
+    for (uint32_t e = threadIdx.x; e < tl.count; e += kKarrasThreads) {
       const int i = (int)(tl.start + e);
       if (i >= n - 1) continue;
       const int dir = kdelta(kw, n, i, i + 1) > kdelta(kw, n, i, i - 1) ? 1 : -1;
@@ -1794,9 +1798,11 @@ hipError_t launch_flatten(hipStream_t st, const SegDesc* segs, const Tile* et, u
 hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net) {
   if (md.key63)
-    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(grid_for(net)), dim3(kKarrasThreads), 0, st, segs,
+                       et, net);
   else
-    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(grid_for(net)), dim3(kKarrasThreads), 0, st, segs,
+                       et, net);
   return hipGetLastError();
 }
 
