@@ -478,14 +478,17 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
 // ----------------------------------------------------------------------------------
 
 // Sorted leaves of a fresh build, or the positions of a cached tree refreshed in its
-// existing leaf order (the same arithmetic).
-__global__ void __launch_bounds__(kBlock)
+// existing leaf order (the same arithmetic). 1024 threads per 2048-element tile keep more
+// of the random reads in flight.
+constexpr int kGatherThreads = 1024;
+
+__global__ void __launch_bounds__(kGatherThreads)
 k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const double o0 = s.bounds[5], o1 = s.bounds[6], o2 = s.bounds[7];
-    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
+    for (uint32_t e = threadIdx.x; e < tl.count; e += kGatherThreads) {
       const uint32_t k = tl.start + e;
       const size_t i = min(s.vals[0][k], s.n - 1);
       float4 lf;
@@ -1779,13 +1782,13 @@ hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint
 }
 
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kGatherThreads), 0, st, segs, et, net);
   return hipGetLastError();
 }
 
 hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
                                  uint32_t net) {
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kBlock), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kGatherThreads), 0, st, segs, et, net);
   return hipGetLastError();
 }
 
