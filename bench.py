@@ -59,9 +59,9 @@ FLOPS_L2P = 60
 # `bench.py --profile --leaf-size 1 --far-theta 0.55 --mono-ratio 0.8 --group-size 64
 # --group-lanes 4 --radix-bits 8` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled by the same
 # counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE reads half
-# the copied bytes (x2), WRITE_SIZE all of them (x1). 43.59 GB fetched + 0.56 GB written;
+# the copied bytes (x2), WRITE_SIZE all of them (x1). 43.57 GB fetched + 0.56 GB written;
 # the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 43.594e9 + 0.558e9
+WALK_TRAFFIC_BYTES = 43.573e9 + 0.558e9
 # (leaf size, far_theta, mono_ratio, group size, lanes per body) the walk counters were
 # measured on
 WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4)
