@@ -66,14 +66,14 @@ struct glx_solver {
   Batch* batch = nullptr;
   uint32_t n_cur = 0xFFFFFFFFu;
   glx_stats stats{};
-  double* pot_partial = nullptr;
-  size_t pot_cap = 0;
+  double* pot_buf = nullptr;  // potential_energy's own x[3n], m[n] and block partials
+  size_t pot_cap = 0;         // doubles
   uint8_t* mask = nullptr;  // walk mask of the active calls (original order)
   size_t mask_cap = 0;
   uint32_t cache_n = 0;     // bodies of the tree kept by the last build (0: none)
   ~glx_solver() {
     delete batch;
-    (void)hipFree(pot_partial);
+    (void)hipFree(pot_buf);
     (void)hipFree(mask);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -112,6 +112,7 @@ static void solver_stage_inputs(glx_solver* s, size_t n, const double* pos, cons
   if (s->n_cur != n || s->seg.capacity != cap_before) {
     s->batch->set_segments({s->seg.desc});
     s->n_cur = (uint32_t)n;
+    s->cache_n = 0;  // the kept tree described other storage or another n
   }
   h2d(s->seg.desc.x, pos, 3 * n * sizeof(double), s->st);
   h2d(s->seg.desc.m, mass, n * sizeof(double), s->st);
@@ -267,19 +268,27 @@ extern "C" int glx_solver_potential_energy(glx_solver* s, size_t n, const double
     *out = 0.0;
     if (n < 2) return;
     if (!pos || !mass) fail(GLX_EINVAL, "NULL array");
-    solver_stage_inputs(s, n, pos, mass);
+    // The potential works on its own copy of the bodies, so the solver's segment and
+    // the tree kept for gravity_active_cached stay as the last force evaluation left
+    // them (the reference's potential_energy takes &self).
     const size_t nb = (n + 255) / 256 + 1;
-    if (nb > s->pot_cap) {
-      (void)hipFree(s->pot_partial);
-      s->pot_partial = nullptr;
-      check(hipMalloc(&s->pot_partial, nb * sizeof(double)), "hipMalloc(potential)");
-      s->pot_cap = nb;
+    const size_t need = 4 * n + nb;
+    if (need > s->pot_cap) {
+      (void)hipFree(s->pot_buf);
+      s->pot_buf = nullptr;
+      s->pot_cap = 0;
+      check(hipMalloc(&s->pot_buf, need * sizeof(double)), "hipMalloc(potential)");
+      s->pot_cap = need;
     }
-    check(launch_potential_f64(s->st, s->seg.desc.x, s->seg.desc.m, (uint32_t)n, s->p.g,
-                               s->p.softening * s->p.softening, s->pot_partial,
-                               s->pot_partial + (nb - 1)),
+    double* px = s->pot_buf;
+    double* pm = px + 3 * n;
+    double* part = pm + n;
+    h2d(px, pos, 3 * n * sizeof(double), s->st);
+    h2d(pm, mass, n * sizeof(double), s->st);
+    check(launch_potential_f64(s->st, px, pm, (uint32_t)n, s->p.g,
+                               s->p.softening * s->p.softening, part, part + (nb - 1)),
           "potential");
-    d2h(out, s->pot_partial + (nb - 1), sizeof(double), s->st);
+    d2h(out, part + (nb - 1), sizeof(double), s->st);
     sync(s->st);
   });
 }

@@ -49,7 +49,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
-// Block-wide exclusive scan of one u32 per thread (blockDim == kBlock == 256).
+// Block-wide exclusive scan of one u32 per thread, any blockDim that is a multiple of
+// 64; sh_wave must hold blockDim / 64 entries (one per wave).
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh_wave) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t inc = v;
@@ -387,7 +388,10 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
   __shared__ uint32_t dstart[BINS];
   __shared__ K skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
-  __shared__ uint32_t shw[W];
+  __shared__ uint32_t shw[W];  // block_exclusive_scan's per-wave totals
+  static_assert(kSortThreads % 64 == 0 && W * 64 == kSortThreads,
+                "one shw entry per wave of the block");
+  static_assert(R * kSortThreads >= BINS, "the tile scan covers every digit");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long lt = (1ull << lane) - 1ull;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {

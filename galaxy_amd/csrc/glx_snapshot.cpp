@@ -23,6 +23,39 @@ constexpr uint32_t kVersion = 4;
 constexpr uint32_t kMinVersion = 1;
 constexpr size_t kMaxStr = 1u << 16;
 
+// Well-formed UTF-8 as Rust's String::from_utf8 accepts it: shortest-form sequences of
+// 1-4 bytes, no surrogates (U+D800..U+DFFF), nothing above U+10FFFF.
+bool valid_utf8(const unsigned char* s, size_t len) {
+  size_t i = 0;
+  while (i < len) {
+    const unsigned c = s[i];
+    if (c < 0x80) {
+      ++i;
+      continue;
+    }
+    int extra;
+    unsigned cp, lo;
+    if (c >= 0xC2 && c <= 0xDF) {
+      extra = 1, cp = c & 0x1F, lo = 0x80;
+    } else if (c >= 0xE0 && c <= 0xEF) {
+      extra = 2, cp = c & 0x0F, lo = 0x800;
+    } else if (c >= 0xF0 && c <= 0xF4) {
+      extra = 3, cp = c & 0x07, lo = 0x10000;
+    } else {
+      return false;  // a continuation byte, an overlong lead (C0, C1) or F5..FF
+    }
+    if (len - i <= (size_t)extra) return false;
+    for (int k = 1; k <= extra; ++k) {
+      const unsigned t = s[i + k];
+      if ((t & 0xC0) != 0x80) return false;
+      cp = (cp << 6) | (t & 0x3F);
+    }
+    if (cp < lo || cp > 0x10FFFF || (cp >= 0xD800 && cp <= 0xDFFF)) return false;
+    i += 1 + extra;
+  }
+  return true;
+}
+
 struct Out {
   std::vector<unsigned char> b;
   template <class T>
@@ -34,6 +67,9 @@ struct Out {
   void str(const char* s) {
     const size_t len = s ? std::strlen(s) : 0;
     if (len > kMaxStr) fail(GLX_EINVAL, "header string exceeds 65536 bytes");
+    // the reference writes Rust &str values, which are always UTF-8
+    if (!valid_utf8(reinterpret_cast<const unsigned char*>(s), len))
+      fail(GLX_EINVAL, "header string is not valid UTF-8");
     put<uint32_t>((uint32_t)len);
     b.insert(b.end(), s, s + len);
   }
@@ -55,6 +91,8 @@ struct In {
     const uint32_t len = get<uint32_t>();
     if (len > kMaxStr) fail(GLX_EFORMAT, "header string length exceeds 65536 bytes");
     if (left < len) fail(GLX_EFORMAT, "truncated snapshot stream");
+    // io/src/snapshot.rs read_str decodes with String::from_utf8 and rejects the rest
+    if (!valid_utf8(p, len)) fail(GLX_EFORMAT, "header string is not valid UTF-8");
     std::string s(reinterpret_cast<const char*>(p), len);
     p += len;
     left -= len;

@@ -124,6 +124,11 @@ def reference_morton(pos):
     return {"bmin": bounds[:3].copy(), "size": bounds[3], "scale": bounds[4]}, lanes, codes
 
 
+def expand21(v: int) -> int:
+    """The 21-bit lane spread of the 63-bit keys (bit b of v -> bit 3b)."""
+    return int(lib().orc_expand21(v & 0xFFFFFFFFFFFFFFFF))
+
+
 def reference_morton63(pos):
     """63-bit variant of solvers/src/lbvh.rs:105-129 (2^21 cells per axis)."""
     pos = _f64(pos).reshape(-1, 3)

@@ -80,3 +80,24 @@ def test_errors():
         s.accelerations_active(pos, mass, [0, 500])
     assert e.value.code == GLX_EINVAL  # index out of range
     assert np.array_equal(s.accelerations_active(pos[:1], mass[:1], [0]), np.zeros((1, 3)))
+
+
+@pytest.mark.parametrize("mode", ["reference", "fast"])
+def test_potential_energy_leaves_the_cached_tree_alone(mode):
+    """potential_energy takes &self in the reference (core/src/traits.rs:95): a call on
+    other bodies between rebuild_gravity_cache and gravity_active_cached, with more
+    bodies than the solver's storage holds, must not disturb the kept tree."""
+    pos, _, mass = ic.plummer(6000, 11)
+    big, _, bmass = ic.plummer(30000, 12)
+    act = np.arange(0, 6000, 5)
+    ref = glx.Lbvh(1.0, 0.01, 0.5, **MODES[mode])
+    ref.rebuild_gravity_cache(pos, mass)
+    want = ref.gravity_active_cached(pos, mass, act)[act]
+    s = glx.Lbvh(1.0, 0.01, 0.5, **MODES[mode])
+    s.rebuild_gravity_cache(pos, mass)
+    u_big = s.potential_energy(big, bmass)
+    assert np.array_equal(s.gravity_active_cached(pos, mass, act)[act], want)
+    u_small = s.potential_energy(pos[:3000], mass[:3000])
+    assert np.array_equal(s.gravity_active_cached(pos, mass, act)[act], want)
+    assert u_big < 0 and u_small < 0
+    assert u_big == s.potential_energy(big, bmass)

@@ -2,8 +2,10 @@
 wave-cooperative group walk.
 
 - 63-bit codes, sort and topology are bit-exact with the oracle's 63-bit restatement
-  (oracle/lbvh_oracle.c orc_reference_morton63/sort63/karras63, itself pinned against an
-  independent numpy restatement and a brute-force radix tree in tests/test_oracle.py).
+  (oracle/lbvh_oracle.c orc_expand21/reference_morton63/sort63/karras63, itself pinned in
+  tests/test_oracle.py against an independent numpy restatement, a mask-based bit spread,
+  numpy's stable argsort, a brute-force radix tree, and the exact reductions
+  code63 >> 33 == code30 and karras63(code30 << 33) == karras30(code30)).
 - Quadrupole moments (no reference counterpart: the reference is monopole-only,
   solvers/src/barnes_hut.rs:33) are checked against brute-force sums over each node's
   leaves, and through the force error they buy against an FP64 direct sum.

@@ -200,6 +200,182 @@ def test_karras_known_answers():
     assert list(p) == ka["parent"]
 
 
+# ----------------------------------------------------------------- 63-bit keys
+# The 63-bit variant (2^21 cells per axis) is the reference's deferred refinement
+# (solvers/src/lbvh.rs:29-31) over the cube convention of :105-129. Each function is
+# pinned three ways: an independent numpy restatement, a different construction (a
+# magic-mask bit spread, numpy's stable argsort, the brute-force top-down radix tree),
+# and an exact reduction to the already pinned 30-bit path.
+
+
+def _expand21_masks(v: int) -> int:
+    """Bit spread by shift-and-mask doubling (a different construction from the oracle's
+    per-bit loop)."""
+    x = v & 0x1FFFFF
+    x = (x | (x << 32)) & 0x001F00000000FFFF
+    x = (x | (x << 16)) & 0x001F0000FF0000FF
+    x = (x | (x << 8)) & 0x100F00F00F00F00F
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3
+    x = (x | (x << 2)) & 0x1249249249249249
+    return x
+
+
+def test_expand21_known_answers_and_mask_construction():
+    assert O.expand21(0) == 0
+    assert O.expand21(1) == 1
+    assert O.expand21(2) == 8
+    assert O.expand21(0x1FFFFF) == 0x1249249249249249
+    assert O.expand21(1 << 20) == 1 << 60
+    assert O.expand21(0x3FFFFF) == O.expand21(0x1FFFFF)  # bits above 21 are dropped
+    rng = np.random.default_rng(21)
+    sample = np.concatenate([np.arange(4096), rng.integers(0, 1 << 21, 20000),
+                             (1 << 21) - 1 - np.arange(4096)])
+    for v in sample:
+        v = int(v)
+        assert O.expand21(v) == _expand21_masks(v), v
+
+
+@pytest.mark.parametrize("seed", [1, 7, 42, 1000])
+def test_morton63_matches_numpy_restatement(seed):
+    pos = I.cloud(seed, 8000, 3.0)
+    b, lanes, codes = O.reference_morton63(pos)
+    b2, lanes2, codes2 = R.morton63(pos)
+    assert b["size"] == b2["size"] and b["scale"] == b2["scale"]
+    assert np.array_equal(b["bmin"], b2["bmin"])
+    assert np.array_equal(lanes, lanes2)
+    assert np.array_equal(codes, codes2)
+    assert lanes.max() < (1 << 21) and codes.max() < (1 << 63)
+
+
+def test_morton63_large_coordinates_match_numpy_restatement():
+    pos = I.cloud(3, 8000, 3.0, (1.0e6, 1.0e6, 1.0e6))
+    _, lanes, codes = O.reference_morton63(pos)
+    _, lanes2, codes2 = R.morton63(pos)
+    assert np.array_equal(lanes, lanes2) and np.array_equal(codes, codes2)
+
+
+@pytest.mark.parametrize("seed", [1, 7, 42])
+def test_morton63_refines_the_30bit_codes(seed):
+    """2^21 / size is 2^11 · (1024 / size) exactly (a power-of-two scaling commutes with
+    rounding), so every 21-bit lane shifted right by 11 is the 30-bit lane, both clamps
+    agree, and the 63-bit code shifted right by 33 is the reference's 30-bit code."""
+    for pos in (I.cloud(seed, 6000, 3.0), I.cloud(seed, 3000, 3.0, (1.0e6, -2.0e5, 7.0)),
+                I.plummer_numpy(4000, seed=seed)[0]):
+        b30, lanes30, codes30 = O.reference_morton(pos)
+        b63, lanes63, codes63 = O.reference_morton63(pos)
+        assert np.array_equal(b30["bmin"], b63["bmin"]) and b30["size"] == b63["size"]
+        assert b63["scale"] == 2048.0 * b30["scale"]
+        assert np.array_equal(lanes63 >> 11, lanes30)
+        assert np.array_equal((codes63 >> np.uint64(33)).astype(np.uint32), codes30)
+        for l, c in zip(lanes63[:300], codes63[:300]):
+            want = 0
+            for a in range(3):
+                want |= _expand21_masks(int(l[a])) << a
+            assert int(c) == want
+
+
+def test_morton63_collinear_single_and_coincident():
+    n = KA["bbox_collinear"]["n"]
+    pos = np.array([[i * 0.1, 2.0, -1.0] for i in range(n)])
+    _, lanes, _ = O.reference_morton63(pos)
+    assert lanes[0, 0] == 0 and lanes[-1, 0] == (1 << 21) - 1
+    assert np.all(np.diff(lanes[:, 0].astype(np.int64)) > 0)
+    for a in (1, 2):
+        assert np.all(lanes[:, a] == lanes[0, a])
+        assert lanes[0, a] >> 11 in (511, 512)
+    # a lone particle sits at the centre of a 2e-12 cube: f64 rounding of p - bmin at
+    # |p| ~ 1 moves it by many of the 2^21 fine cells, but not out of the coarse cell 512
+    _, lanes, _ = O.reference_morton63(np.array(KA["single_particle"]["pos"]))
+    assert np.all(lanes >> 11 == 512)
+    ka = KA["coincident_share_code"]
+    _, lanes, codes = O.reference_morton63(np.tile(np.array(ka["pos"]), (ka["n"], 1)))
+    assert np.all(codes == codes[0]) and np.all(lanes == lanes[0])
+    with pytest.raises(ValueError):
+        O.reference_morton63(np.zeros((0, 3)))
+
+
+def _codes63(seed, n):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 1 << 63, n, dtype=np.uint64)
+
+
+@pytest.mark.parametrize(
+    "codes",
+    [
+        _codes63(2, 10000),
+        _codes63(3, 65536),
+        (np.arange(5000, dtype=np.uint64) % 4) << np.uint64(40),
+        # keys equal in the low 32 bits and different above them: a sort that drops the
+        # high word fails here
+        (np.arange(4096, dtype=np.uint64)[::-1] << np.uint64(33)) | np.uint64(0x5A5A5A5A),
+        # keys equal above bit 32 and different below
+        np.uint64(0x7000000000000000) | (_codes63(4, 4096) & np.uint64(0xFFFFFFFF)),
+        np.full(3000, (1 << 63) - 1, np.uint64),
+    ],
+    ids=["random", "large", "ties4_high", "high_word_only", "low_word_only", "all_equal"],
+)
+def test_sort63_matches_stable_argsort(codes):
+    assert np.array_equal(O.reference_sort63(codes),
+                          np.argsort(codes, kind="stable").astype(np.uint32))
+
+
+def test_sort63_known_answers():
+    assert np.array_equal(O.reference_sort63(np.arange(2000, dtype=np.uint64)), np.arange(2000))
+    assert np.array_equal(O.reference_sort63(np.arange(2000, dtype=np.uint64)[::-1]),
+                          np.arange(2000)[::-1])
+    assert list(O.reference_sort63(np.array([9], np.uint64))) == [0]
+    assert O.reference_sort63(np.zeros(0, np.uint64)).size == 0
+
+
+def _check_karras63_vs_bruteforce(sc):
+    l1, r1, p1 = O.reference_karras63(sc)
+    l2, r2, p2 = R.radix_tree_bruteforce63(sc)
+    assert np.array_equal(l1, l2) and np.array_equal(r1, r2) and np.array_equal(p1, p2)
+    _check_structural(l1, r1, p1, len(sc))
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 17, 128, 257, 2000])
+def test_karras63_matches_bruteforce_on_morton_codes(n):
+    for seed in (1, 7):
+        _, _, codes = O.reference_morton63(I.cloud(seed, n, 3.0))
+        _check_karras63_vs_bruteforce(np.sort(codes))
+
+
+def test_karras63_all_equal_duplicates_chain_and_word_boundaries():
+    _check_karras63_vs_bruteforce(np.full(300, 42, np.uint64))
+    _check_karras63_vs_bruteforce(np.full(200, (1 << 63) - 1, np.uint64))
+    _check_karras63_vs_bruteforce((np.arange(2000, dtype=np.uint64) // 100) << np.uint64(35))
+    _check_karras63_vs_bruteforce(np.arange(1500, dtype=np.uint64))
+    # differences only above bit 32, only below it, and straddling it
+    _check_karras63_vs_bruteforce(np.sort(_codes63(5, 1000) & np.uint64(0x7FFFFFFF00000000)))
+    _check_karras63_vs_bruteforce(np.sort(_codes63(6, 1000) & np.uint64(0x00000000FFFFFFFF)))
+    _check_karras63_vs_bruteforce(np.sort(_codes63(7, 1000) & np.uint64(0x0000000FFFF00000)))
+
+
+def test_karras63_known_answers():
+    l, r, p = O.reference_karras63(np.array([5], np.uint64))
+    assert list(p) == KA["tree_single"]["parent"] and l.size == 0
+    ka = KA["tree_two"]
+    _, _, codes = O.reference_morton63(np.array(ka["pos"]))
+    order = O.reference_sort63(codes)
+    _, _, p = O.reference_karras63(codes[order])
+    assert list(p) == ka["parent"]
+
+
+@pytest.mark.parametrize("seed", [1, 7, 42])
+def test_karras63_of_shifted_30bit_codes_is_the_30bit_tree(seed):
+    """For codes c << 33, δ63 maps the 30-bit δ order-preservingly (distinct codes:
+    δ30 − 1; ties: δ30 + 32; out of range: −1), so the Karras search takes the same
+    branches and the topology equals reference_karras on c."""
+    for n in (1000, 20000):
+        _, _, codes = O.reference_morton(I.cloud(seed, n, 3.0))
+        sc = np.sort(codes)
+        l30, r30, p30 = O.reference_karras(sc)
+        l63, r63, p63 = O.reference_karras63(sc.astype(np.uint64) << np.uint64(33))
+        assert np.array_equal(l30, l63) and np.array_equal(r30, r63)
+        assert np.array_equal(p30, p63)
+
+
 # ----------------------------------------------------------------- reference_aggregate
 
 

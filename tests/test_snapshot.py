@@ -172,3 +172,39 @@ def test_invalid_species_and_long_strings(tmp_path):
     bad = dict(cols, kind=np.full(mass.size, 2, np.uint8))
     assert _code(lambda: sim.write_snapshot(tmp_path / "m.snap", pos, vel, mass, **HDR,
                                             **bad)) == GLX_EINVAL
+
+
+def _with_units(pos, vel, mass, cols, units: bytes):
+    raw = expected_bytes(pos, vel, mass, cols, **HDR)
+    at = 12 + 56  # magic + version, then the 56-byte fixed header
+    old = len(HDR["units"].encode())
+    return raw[:at] + struct.pack("<I", len(units)) + units + raw[at + 4 + old:]
+
+
+@pytest.mark.parametrize("units", [b"\xff", b"ab\xc0\xaf", b"\xed\xa0\x80", b"\xf4\x90\x80\x80",
+                                   b"x\xe2\x82", b"\x80abc"],
+                         ids=["ff", "overlong", "surrogate", "above_10ffff", "cut", "lone_cont"])
+def test_header_strings_must_be_utf8(tmp_path, units):
+    """io/src/snapshot.rs read_str decodes with String::from_utf8: a stream whose header
+    strings are not UTF-8 is refused, and the writer (which the reference feeds &str) will
+    not produce one."""
+    import ctypes as C
+    import os
+
+    pos, vel, mass, cols = sample()
+    p = tmp_path / "u.snap"
+    p.write_bytes(_with_units(pos, vel, mass, cols, units))
+    assert _code(lambda: sim.read_snapshot(p)) == sim.GLX_EFORMAT
+    h = sim.SnapHeader(1.0, 3, 1.0, 0.05, 0, 1, 2, units, b"v")
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    q = tmp_path / "w.snap"
+    assert sim._bind().glx_snapshot_write(os.fsencode(q), C.byref(h), mass.size, ptr(pos),
+                                          ptr(vel), ptr(mass), None) == GLX_EINVAL
+
+
+def test_multibyte_utf8_header_strings_round_trip(tmp_path):
+    pos, vel, mass, cols = sample()
+    units = "Msun·kpc·Gyr — ∑ 𝔊"
+    p = tmp_path / "m.snap"
+    p.write_bytes(_with_units(pos, vel, mass, cols, units.encode()))
+    assert sim.read_snapshot(p).units == units
