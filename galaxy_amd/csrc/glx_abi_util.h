@@ -84,6 +84,17 @@ inline void validate_n(size_t n) {
   if (n > kMaxN) fail(GLX_EINVAL, "n exceeds 2^29 particles per system");
 }
 
+// Arguments of the FP64 direct sum: targets index the n sources.
+inline void validate_direct(size_t n, const double* pos, const double* mass,
+                            const uint32_t* targets, size_t n_targets, double* acc_out) {
+  validate_n(n);
+  if (n_targets == 0) return;
+  if (!pos || !mass || !targets || !acc_out) fail(GLX_EINVAL, "NULL array");
+  if (n_targets > 0xFFFFFFFFu) fail(GLX_EINVAL, "too many targets");
+  for (size_t t = 0; t < n_targets; ++t)
+    if (targets[t] >= n) fail(GLX_EINVAL, "target index out of range");
+}
+
 inline void fill_params(SegDesc& d, const glx_params& p, uint32_t n) {
   d.n = n;
   d.g = (float)p.g;

@@ -297,20 +297,19 @@ extern "C" int glx_direct_sum_f64(size_t n, const double* pos, const double* mas
                                   double softening, const uint32_t* targets, size_t n_targets,
                                   double* acc_out) {
   return guard([&] {
-    validate_n(n);
+    validate_direct(n, pos, mass, targets, n_targets, acc_out);
     if (n_targets == 0) return;
-    if (!pos || !mass || !targets || !acc_out) fail(GLX_EINVAL, "NULL array");
-    for (size_t t = 0; t < n_targets; ++t)
-      if (targets[t] >= n) fail(GLX_EINVAL, "target index out of range");
     ensure_device();
+    const DirectPlan plan = direct_plan((uint32_t)n, (uint32_t)n_targets, kDirectPartialBytes);
     Stream S;
     DevArray<double> x(3 * n), m(n), out(3 * n_targets);
+    DevArray<double> part(plan.chunks > 1 ? 3 * n_targets * plan.chunks : 0);
     DevArray<uint32_t> tg(n_targets);
     h2d(x.p, pos, 3 * n * sizeof(double), S.st);
     h2d(m.p, mass, n * sizeof(double), S.st);
     h2d(tg.p, targets, n_targets * sizeof(uint32_t), S.st);
     check(launch_direct_f64(S.st, x.p, m.p, (uint32_t)n, tg.p, (uint32_t)n_targets, g,
-                            softening * softening, out.p),
+                            softening * softening, plan, part.p, out.p),
           "direct_f64");
     d2h(acc_out, out.p, 3 * n_targets * sizeof(double), S.st);
     sync(S.st);

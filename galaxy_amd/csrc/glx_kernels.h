@@ -36,9 +36,17 @@ hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, cons
                        uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count);
 hipError_t launch_diag(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
                        uint32_t net, double* partial, double* out);
+// FP64 direct sum onto nt targets: the sources split into `chunks` ranges of chunk_len
+// (a multiple of the 256-source LDS tile); with two or more chunks the per-chunk sums go
+// to partial (3 * nt * chunks doubles) and are added in chunk order.
+struct DirectPlan {
+  uint32_t chunks, chunk_len;
+};
+constexpr size_t kDirectPartialBytes = (size_t)16 << 20;  // partial sums of one call, at most
+DirectPlan direct_plan(uint32_t n, uint32_t nt, size_t partial_bytes);
 hipError_t launch_direct_f64(hipStream_t st, const double* x, const double* m, uint32_t n,
                              const uint32_t* targets, uint32_t nt, double g, double eps2,
-                             double* out);
+                             const DirectPlan& plan, double* partial, double* out);
 hipError_t launch_potential_f64(hipStream_t st, const double* x, const double* m, uint32_t n,
                                 double g, double eps2, double* partial, double* out);
 

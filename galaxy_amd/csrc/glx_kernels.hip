@@ -1597,27 +1597,33 @@ __global__ void k_diag_final(const SegDesc* __restrict__ segs, uint32_t nseg,
 
 constexpr int kDsTile = 256;
 
+// Targets [blockIdx.x * 256, +256) against the sources of chunk blockIdx.y, [c * len,
+// min(n, (c + 1) * len)), self excluded. With one chunk the sums go straight to out
+// (3 per target); with several, chunk c writes partial[(c * nt + t) * 3 + axis] and
+// k_direct_reduce adds the chunks in index order, so results do not depend on timing.
 __global__ void __launch_bounds__(kDsTile)
 k_direct_f64(const double* __restrict__ x, const double* __restrict__ m, uint32_t n,
              const uint32_t* __restrict__ targets, uint32_t nt, double g, double eps2,
-             double* __restrict__ out) {
+             uint32_t chunk_len, double* __restrict__ out) {
   __shared__ double sx[kDsTile], sy[kDsTile], sz[kDsTile], sm[kDsTile];
   const uint32_t t = blockIdx.x * kDsTile + threadIdx.x;
   const bool act = t < nt;
   const uint32_t i = act ? targets[t] : 0u;
   const double xi = act ? x[3 * (size_t)i] : 0.0, yi = act ? x[3 * (size_t)i + 1] : 0.0,
                zi = act ? x[3 * (size_t)i + 2] : 0.0;
+  const uint32_t j0 = blockIdx.y * chunk_len;
+  const uint32_t j1 = (n - j0 < chunk_len) ? n : j0 + chunk_len;
   double ax = 0.0, ay = 0.0, az = 0.0;
-  for (uint32_t base = 0; base < n; base += kDsTile) {
+  for (uint32_t base = j0; base < j1; base += kDsTile) {
     const uint32_t j = base + threadIdx.x;
-    if (j < n) {
+    if (j < j1) {
       sx[threadIdx.x] = x[3 * (size_t)j];
       sy[threadIdx.x] = x[3 * (size_t)j + 1];
       sz[threadIdx.x] = x[3 * (size_t)j + 2];
       sm[threadIdx.x] = m[j];
     }
     __syncthreads();
-    const uint32_t cnt = (n - base < (uint32_t)kDsTile) ? n - base : (uint32_t)kDsTile;
+    const uint32_t cnt = (j1 - base < (uint32_t)kDsTile) ? j1 - base : (uint32_t)kDsTile;
     for (uint32_t q = 0; q < cnt; ++q) {
       const double dx = sx[q] - xi, dy = sy[q] - yi, dz = sz[q] - zi;
       const double r2 = dx * dx + dy * dy + dz * dz + eps2;
@@ -1630,10 +1636,22 @@ k_direct_f64(const double* __restrict__ x, const double* __restrict__ m, uint32_
     __syncthreads();
   }
   if (act) {
-    out[3 * (size_t)t] = ax;
-    out[3 * (size_t)t + 1] = ay;
-    out[3 * (size_t)t + 2] = az;
+    double* o = out + ((size_t)blockIdx.y * nt + t) * 3;
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = az;
   }
+}
+
+// out[3 t + axis] = sum over chunks c = 0, 1, ... of partial[(c * nt + t) * 3 + axis].
+__global__ void __launch_bounds__(kDsTile)
+k_direct_reduce(const double* __restrict__ partial, uint32_t nt, uint32_t chunks,
+                double* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * kDsTile + threadIdx.x;
+  if (e >= 3 * (size_t)nt) return;
+  double r = 0.0;
+  for (uint32_t c = 0; c < chunks; ++c) r += partial[(size_t)c * 3 * nt + e];
+  out[e] = r;
 }
 
 // Softened potential: rows [blockIdx*256, ...) against all j > i; one partial per block.
@@ -1874,12 +1892,41 @@ hipError_t launch_diag(hipStream_t st, const SegDesc* segs, uint32_t nseg, const
   return hipGetLastError();
 }
 
+DirectPlan direct_plan(uint32_t n, uint32_t nt, size_t partial_bytes) {
+  DirectPlan p{1u, n ? n : 1u};
+  if (nt == 0 || n == 0) return p;
+  // about 2048 blocks of 256 targets x one source chunk fill the 256 CUs several times
+  const uint32_t tblocks = (nt + kDsTile - 1) / kDsTile;
+  uint32_t c = (2048u + tblocks - 1) / tblocks;
+  const uint32_t by_src = (n + kDsTile - 1) / kDsTile;  // at least one LDS tile per chunk
+  if (c > by_src) c = by_src;
+  if (c > 256u) c = 256u;
+  const size_t fit = partial_bytes / (3 * sizeof(double) * (size_t)nt);
+  if ((size_t)c > fit) c = (uint32_t)fit;
+  if (c < 2u) return p;
+  uint32_t len = (n + c - 1) / c;
+  len = (len + kDsTile - 1) / kDsTile * kDsTile;
+  p.chunks = (n + len - 1) / len;
+  p.chunk_len = len;
+  if (p.chunks < 2u) p = DirectPlan{1u, n};
+  return p;
+}
+
 hipError_t launch_direct_f64(hipStream_t st, const double* x, const double* m, uint32_t n,
                              const uint32_t* targets, uint32_t nt, double g, double eps2,
-                             double* out) {
+                             const DirectPlan& plan, double* partial, double* out) {
   if (nt == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_direct_f64, dim3((nt + kDsTile - 1) / kDsTile), dim3(kDsTile), 0, st, x, m,
-                     n, targets, nt, g, eps2, out);
+  const uint32_t tblocks = (nt + kDsTile - 1) / kDsTile;
+  if (plan.chunks < 2u) {
+    hipLaunchKernelGGL(k_direct_f64, dim3(tblocks, 1), dim3(kDsTile), 0, st, x, m, n, targets,
+                       nt, g, eps2, plan.chunk_len, out);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_direct_f64, dim3(tblocks, plan.chunks), dim3(kDsTile), 0, st, x, m, n,
+                     targets, nt, g, eps2, plan.chunk_len, partial);
+  const uint32_t rblocks = (uint32_t)((3 * (size_t)nt + kDsTile - 1) / kDsTile);
+  hipLaunchKernelGGL(k_direct_reduce, dim3(rblocks), dim3(kDsTile), 0, st, partial, nt,
+                     plan.chunks, out);
   return hipGetLastError();
 }
 
