@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgalaxy_lbvh.so")
 SOURCES = [
-    "glx_kernels.hip", "glx_runtime.hip", "glx_api.hip", "glx_pool.hip", "glx_ic.hip",
+    "glx_kernels.hip", "glx_runtime.hip", "glx_api.hip", "glx_pool.hip", "glx_ic.hip", "glx_noalloc.hip",
     "glx_snapshot.cpp",
 ]
 

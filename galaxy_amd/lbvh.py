@@ -109,6 +109,7 @@ def lib():
         "glx_solver_potential_energy": (I, [P, S, P, P, C.POINTER(D)]),
         "glx_solver_stats": (I, [P, C.POINTER(Stats)]),
         "glx_direct_sum_f64": (I, [S, P, P, D, D, P, S, P]),
+        "glx_direct_sum_f64_noalloc": (I, [S, P, P, D, D, P, S, P]),
         "glx_stage_morton": (I, [S, P, P, P, P]),
         "glx_stage_morton63": (I, [S, P, P, P, P]),
         "glx_stage_sort": (I, [S, P, P, P]),
@@ -271,14 +272,17 @@ class Lbvh:
         return st.as_dict()
 
 
-def direct_sum_f64(pos, mass, g: float, softening: float, targets) -> np.ndarray:
-    """FP64 direct sum onto targets (solvers/src/direct_sum.rs:25-45 pair kernel)."""
+def direct_sum_f64(pos, mass, g: float, softening: float, targets,
+                   noalloc: bool = False) -> np.ndarray:
+    """FP64 direct sum onto targets (solvers/src/direct_sum.rs:25-45 pair kernel).
+    noalloc: through glx_direct_sum_f64_noalloc (no device allocation; same bits)."""
     pos = _f64(pos, (-1, 3))
     mass = _f64(mass)
     tg = _u32(targets)
     out = np.zeros((tg.size, 3))
-    _check(lib().glx_direct_sum_f64(mass.size, _p(pos), _p(mass), float(g), float(softening),
-                                    _p(tg), tg.size, _p(out)))
+    f = lib().glx_direct_sum_f64_noalloc if noalloc else lib().glx_direct_sum_f64
+    _check(f(mass.size, _p(pos), _p(mass), float(g), float(softening), _p(tg), tg.size,
+             _p(out)))
     return out
 
 

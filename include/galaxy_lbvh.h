@@ -168,6 +168,17 @@ int glx_direct_sum_f64(size_t n, const double *pos, const double *mass, double g
                        double softening, const uint32_t *targets, size_t n_targets,
                        double *acc_out);
 
+/* glx_direct_sum_f64 without any device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight serving loops that check
+ * accuracy between steps, harnesses that forbid hipMalloc). Same arguments and the same
+ * results, bit for bit. Its device memory (staged inputs, output, partial sums) is one
+ * static 64 MiB array per GPU, held until the process ends; GLX_EINVAL when
+ * 32 n + 28 n_targets bytes (each array rounded up to 256 B) exceed its 48 MiB staging
+ * part. Calls on one GPU share the array: they are serialised inside the library. */
+int glx_direct_sum_f64_noalloc(size_t n, const double *pos, const double *mass, double g,
+                               double softening, const uint32_t *targets, size_t n_targets,
+                               double *acc_out);
+
 /* ---------------------------------------------------------------------------------
  * Stage exports: the GPU counterparts of the reference's stage oracles, which it makes
  * public for exactly this gating (solvers/src/lib.rs:15-18).
