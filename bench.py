@@ -62,15 +62,24 @@ FLOPS_L2P = 60
 # the copied bytes (x2), WRITE_SIZE all of them (x1). 43.57 GB fetched + 0.56 GB written;
 # the tested node records come from beyond L2.
 WALK_TRAFFIC_BYTES = 43.573e9 + 0.558e9
-# (leaf size, far_theta, mono_ratio, group size, lanes per body) the walk counters were
+# (leaf size, far_theta, mono_ratio, group size, lanes per body, key bits) the walk counters were
 # measured on
-WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4)
+WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4, 63)
 WALK_VALU_ISSUE = 0.88  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
-PHASE_BYTES_30 = {"kick_drift_bbox": 72, "encode": 32, "sort": 68, "gather": 36, "karras": 20,
-                  "aggregate": 168}
-PHASE_BYTES_63 = {"kick_drift_bbox": 72, "encode": 36, "sort": 196, "gather": 36, "karras": 24,
-                  "aggregate": 264}
+
+
+def phase_bytes(morton_bits, radix_bits, quad):
+    """Algorithmic HBM bytes per particle of each HBM-bound phase: kick-drift reads and
+    writes x (f64) and v (f32); encode reads x and writes key + payload; the LSD sort reads
+    the keys once for the first histogram, then reads and writes key + payload in each of
+    ceil(bits / radix_bits) passes; the gather reads the order and a 16-B leaf source and
+    writes a 16-B leaf; Karras reads the codes and writes children and parents; the
+    bottom-up pass writes and reads back each node record (40 B, 64 B with quadrupoles)."""
+    kb = 4 if morton_bits == 30 else 8
+    passes = -(-(30 if morton_bits == 30 else 63) // (radix_bits or 8))
+    return {"kick_drift_bbox": 72, "encode": 28 + kb, "sort": kb + 2 * (kb + 4) * passes,
+            "gather": 36, "karras": 16 + kb, "aggregate": 264 if quad else 168}
 
 
 HBM_PEAK_GBS = 8000.0
@@ -243,18 +252,20 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
         tg = np.sort(rng.choice(m.size, min(m.size, err_per_system), replace=False))
         errs.append(rel_err(acc[tg], glx.direct_sum_f64(pos, m, 1.0, eps, tg.astype(np.uint32))))
     walk_ms = phases["walk"]
+    bytes_tab = phase_bytes(mode.get("morton_bits"), mode.get("radix_bits"),
+                            mode.get("multipole") == 2)
     flops = (FLOPS_POINT * cnt["point_terms"] + FLOPS_QUAD * cnt["quad_terms"] +
              FLOPS_M2L * cnt["far_nodes"] + (FLOPS_L2P * n_tot if mode.get("far_theta") else 0))
     pool.close()
     return {
         "value": n_tot * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
-        "steps": step_roofline(PHASE_BYTES_63, n_tot, flops, wall * 1e3 / steps),
+        "steps": step_roofline(bytes_tab, n_tot, flops, wall * 1e3 / steps),
         "device_ms": {"p50": float(np.percentile(dev, 50)), "p99": float(np.percentile(dev, 99)),
                       "mean": float(dev.mean())},
         "phase_ms_per_step": phases,
-        "phase_hbm_gbs": {k: PHASE_BYTES_63[k] * n_tot / (phases[k] * 1e-3) / 1e9
-                          for k in PHASE_BYTES_63},
+        "phase_hbm_gbs": {k: bytes_tab[k] * n_tot / (phases[k] * 1e-3) / 1e9
+                          for k in bytes_tab},
         "walk": {"ms": walk_ms, "flops": flops, "tflops": flops / (walk_ms * 1e-3) / 1e12,
                  "i_ref_per_particle": i_ref / n_tot,
                  "flops_ref": FLOPS_POINT * i_ref,
@@ -307,9 +318,8 @@ def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets
     walk_ms = phases["walk"] / steps
     flops = (FLOPS_POINT * c0["point_terms"] + FLOPS_QUAD * c0["quad_terms"] +
              FLOPS_M2L * c0["far_nodes"] + (FLOPS_L2P * n if mode.get("far_theta") else 0))
-    bytes_tab = PHASE_BYTES_63 if mode.get("morton_bits") == 63 else PHASE_BYTES_30
-    if mode.get("multipole") != 2:
-        bytes_tab = dict(bytes_tab, aggregate=168)
+    bytes_tab = phase_bytes(mode.get("morton_bits"), mode.get("radix_bits"),
+                            mode.get("multipole") == 2)
     sysm.close()
     return {
         "snap": snap, "acc0": acc0,
@@ -546,6 +556,8 @@ def main():
                     help="most lanes per body in groups of <= 32 bodies, 1..4 (0: autotune)")
     ap.add_argument("--radix-bits", type=int, default=0,
                     help="sort digit width, 8, 9 or 10 (0: autotune)")
+    ap.add_argument("--morton-bits", type=int, default=0, choices=(0, 30, 63),
+                    help="Morton key width of the ensemble build, 30 or 63 (0: autotune)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
     ap.add_argument("--serve-device-ic", action="store_true",
                     help="serving jobs drawn on the device at admission (glx_pool_admit_ic)")
@@ -613,8 +625,9 @@ def main():
     tuning = None
     leaf, far, mono = args.leaf_size, max(args.far_theta, 0.0), max(args.mono_ratio, 0.0)
     gsize, glanes, rbits = args.group_size or 64, args.group_lanes or 4, args.radix_bits or 8
+    kbits = args.morton_bits or 30
     if not (args.leaf_size and args.far_theta >= 0 and args.mono_ratio >= 0 and
-            args.group_size and args.group_lanes and args.radix_bits):
+            args.group_size and args.group_lanes and args.radix_bits and args.morton_bits):
         from galaxy_amd.tune import GROUP_SHAPES, tune
 
         pick = np.random.default_rng(args.seed + 7).choice(len(systems), 64, replace=False)
@@ -627,16 +640,21 @@ def main():
                       mono_ratios=(args.mono_ratio,) if args.mono_ratio >= 0 else None,
                       group_shapes=shapes,
                       radix_widths=(args.radix_bits,) if args.radix_bits else None,
+                      key_widths=(args.morton_bits,) if args.morton_bits else None,
                       radix_systems=systems)
         leaf, far, mono = tuning["leaf_size"], tuning["far_theta"], tuning["mono_ratio"]
         gsize, glanes, rbits = tuning["group_size"], tuning["group_lanes"], tuning["radix_bits"]
+        kbits = tuning["morton_bits"]
         log(f"tuned in {time.perf_counter() - t0:.1f}s: leaf size {leaf}, far_theta {far}, "
-            f"mono_ratio {mono}, group {gsize}x{glanes} lanes, radix {rbits} bits: "
-            f"{tuning['table']} {tuning['shape_table']} {tuning['radix_table']}")
+            f"mono_ratio {mono}, {kbits}-bit keys, group {gsize}x{glanes} lanes, radix {rbits} "
+            f"bits: {tuning['table']} {tuning['width_table']} {tuning['shape_table']} "
+            f"{tuning['radix_table']}")
+    ens_mode = dict(fast_mode, morton_bits=kbits, leaf_size=leaf, far_theta=far,
+                    mono_ratio=mono, group_size=gsize, group_lanes=glanes, radix_bits=rbits)
     fast_mode.update(leaf_size=leaf, far_theta=far, mono_ratio=mono, group_size=gsize,
                      group_lanes=glanes, radix_bits=rbits)
-    knobs = (leaf, far, mono, gsize, glanes)
-    b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, fast_mode, 8)
+    knobs = (leaf, far, mono, gsize, glanes, kbits)
+    b = batched(glx, Pool, systems, eps, theta, dt, args.steps, args.warmup, ens_mode, 8)
     log(f"batched: {b['value']:.4g} pf/s, {b['ms_per_step']:.2f} ms/step, err {b['error']}")
     out = {
         "metric": METRIC,
@@ -655,8 +673,13 @@ def main():
                                "launch per KDK step",
                    "systems": len(systems), "particles": int(sizes.sum()),
                    "n_range": [1024, 65536], "theta": theta, "eps": eps, "dt": dt,
-                   "seed": args.seed, "build": "63-bit Morton, group walk, quadrupole",
-                   "leaf_size": leaf, "far_theta": far, "mono_ratio": mono,
+                   "seed": args.seed, "build": f"{kbits}-bit Morton, group walk, quadrupole",
+                   # 30-bit keys: the codes, sorted order and tree of every system are those of
+                   # the reference's reference_morton / reference_sort / reference_karras, bit
+                   # for bit (the same kernels as the stage exports the GPU tests check
+                   # against the oracle); 63-bit keys have no reference counterpart
+                   "tree_bit_exact_with_reference": kbits == 30,
+                   "morton_bits": kbits, "leaf_size": leaf, "far_theta": far, "mono_ratio": mono,
                    "group_size": gsize, "group_lanes": glanes, "radix_bits": rbits},
         "steps_per_s": b["steps"]["steps_per_s"],
         # steps/s (and so particle-forces/s) against the whole step's roofline
@@ -695,7 +718,7 @@ def main():
         "tuning": tuning,
     }
     if not args.no_plain:
-        p = plain(glx, systems, eps, theta, dt, args.plain_steps, fast_mode)
+        p = plain(glx, systems, eps, theta, dt, args.plain_steps, ens_mode)
         log(f"plain: {p['value']:.4g} pf/s")
         out["plain"] = p
         out["served_vs_plain"] = b["value"] / p["value"]

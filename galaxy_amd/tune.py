@@ -20,6 +20,14 @@ Walk block shape (error-checked too: the groups' boxes decide which nodes are ac
 * group_lanes: most lanes per body in groups of at most 32 bodies (the lanes of a body
   take every lanes-th list entry and are folded before the store).
 
+Morton key width (30 bits, the reference's codes, or 63 bits, 2^21 cells per axis): with
+30-bit keys the codes, the sorted order and the tree are bit-exact with the reference's
+reference_morton / reference_sort / reference_karras and the sort moves half the bytes;
+63-bit keys resolve clustered inputs whose 30-bit cells hold many bodies (the two-galaxy
+collision's disks). Each width is timed by the whole step and error-checked; among the
+widths within the error bounds the 30-bit build is kept unless the 63-bit one is faster
+by more than `width_margin`.
+
 Sort radix width (8, 9 or 10 bits per LSD pass): fewer passes against more bins per
 tile. The sorted order, hence every result, does not depend on it, so it is timed only.
 
@@ -31,9 +39,9 @@ sum) stays within both bounds: the median within `err_bound` x `margin` (the mar
 absorbs the difference between the sample and the whole workload) and the 99th
 percentile within `p99_bound`. The default
 p99 bound, 1e-2, is below the reference's own per-target monopole walk at theta = 0.5
-(p99 1.4e-2 on a 16k Plummer sphere). The search runs in stages (accuracy knobs, then
-block shape at the chosen knobs, then radix width at the chosen knobs and shape) rather
-than over the full product. `TuneCache` keeps results in a JSON file keyed by the
+(p99 1.4e-2 on a 16k Plummer sphere). The search runs in stages (accuracy knobs, then key
+width, block shape and radix width, each at the knobs chosen before it) rather than over
+the full product. `TuneCache` keeps results in a JSON file keyed by the
 workload's size bucket and theta.
 """
 from __future__ import annotations
@@ -53,6 +61,7 @@ FAR_THETAS = (0.0, 0.5, 0.55, 0.6)
 MONO_RATIOS = (0.0, 0.7, 0.8, 0.9)
 GROUP_SHAPES = ((64, 4), (64, 1), (48, 4), (32, 4))
 RADIX_WIDTHS = (8, 9, 10)
+KEY_WIDTHS = (30, 63)
 
 
 def _pool(systems, knobs, mode, softening, theta, dt):
@@ -138,19 +147,24 @@ def _phase_ms(systems, knobs, mode, softening, theta, dt, steps, phase):
 
 def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e-3,
          p99_bound=1e-2, margin=0.8, per_system=32, seed=5, leaf_sizes=None, far_thetas=None,
-         mono_ratios=None, group_shapes=None, radix_widths=None, radix_systems=None):
-    """All knobs, in three stages on the sample `systems`:
+         mono_ratios=None, group_shapes=None, radix_widths=None, radix_systems=None,
+         key_widths=None, width_margin=0.02):
+    """All knobs, in four stages on the sample `systems`:
     1. leaf size x far_theta x mono_ratio at the default block shape and radix width
-       (`tune_leaf_size`);
-    2. block shape (group_size, group_lanes) at the chosen accuracy knobs: each shape is
+       (`tune_leaf_size`), with 63-bit keys;
+    2. Morton key width (`key_widths`, default 30 and 63) at the chosen accuracy knobs:
+       each width is timed by the median device time of whole pool steps and
+       error-checked; the 30-bit (reference-exact) build is kept unless the 63-bit one
+       is more than `width_margin` faster;
+    3. block shape (group_size, group_lanes) at the chosen knobs and width: each shape is
        timed by its walk phase and error-checked;
-    3. radix width at the chosen knobs and shape, timed by the sort phase on
+    4. radix width at the chosen knobs, key width and shape, timed by the sort phase on
        `radix_systems` (default `systems`; the whole workload where it is at hand: the
        sort's cost per key changes with the number of tiles in flight, and a sample
        fills the GPU less than the workload).
-    Returns {"leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes",
-    "radix_bits", "table", "shape_table", "radix_table"}."""
-    mode = mode or dict(morton_bits=63, walk=lbvh.WALK_GROUP, multipole=2)
+    Returns {"leaf_size", "far_theta", "mono_ratio", "morton_bits", "group_size",
+    "group_lanes", "radix_bits", "table", "width_table", "shape_table", "radix_table"}."""
+    mode = dict(mode or dict(morton_bits=63, walk=lbvh.WALK_GROUP, multipole=2))
     r = tune_leaf_size(systems, softening=softening, theta=theta, dt=dt, mode=mode,
                        steps=steps, err_bound=err_bound, p99_bound=p99_bound, margin=margin,
                        per_system=per_system, seed=seed, leaf_sizes=leaf_sizes,
@@ -159,6 +173,22 @@ def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e
                  mono_ratio=r["mono_ratio"])
     n = sum(m.size for _, _, m in systems)
     picks = _picks(systems, per_system, seed)
+    width_table = []
+    for bits in (key_widths or KEY_WIDTHS):
+        pool, ids, rate, ms = _pool_rate(systems, knobs["leaf_size"], dict(mode, morton_bits=bits),
+                                         softening, theta, dt, steps, knobs["far_theta"],
+                                         knobs["mono_ratio"])
+        med, p99 = _errors(pool, ids, systems, picks, softening)
+        pool.close()
+        width_table.append({"morton_bits": bits, "rate": rate, "device_ms": ms,
+                            "median_err": med, "p99_err": p99})
+    ok = [w for w in width_table
+          if w["median_err"] <= err_bound * margin and w["p99_err"] <= p99_bound] or width_table
+    fastest = max(ok, key=lambda w: w["rate"])
+    exact = [w for w in ok if w["morton_bits"] == 30]
+    width = (exact[0] if exact and exact[0]["rate"] * (1.0 + width_margin) >= fastest["rate"]
+             else fastest)
+    mode["morton_bits"] = width["morton_bits"]
     shape_table = []
     for gs, lanes in (group_shapes or GROUP_SHAPES):
         k = dict(knobs, group_size=gs, group_lanes=lanes)
@@ -176,17 +206,20 @@ def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e
         pool.close()
         radix_table.append({"radix_bits": bits, "sort_ms": ms})
     radix = min(radix_table, key=lambda row: row["sort_ms"])
-    out = dict(knobs, radix_bits=radix["radix_bits"])
-    out.update(table=r["table"], shape_table=shape_table, radix_table=radix_table)
+    out = dict(knobs, morton_bits=mode["morton_bits"], radix_bits=radix["radix_bits"])
+    out.update(table=r["table"], width_table=width_table, shape_table=shape_table,
+               radix_table=radix_table)
     return out
 
 
-KNOBS = ("leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes", "radix_bits")
+KNOBS = ("leaf_size", "far_theta", "mono_ratio", "morton_bits", "group_size", "group_lanes",
+         "radix_bits")
 
 
 class TuneCache:
     """Tuning results in a JSON file, keyed by the workload's size bucket (log2 of the
-    total and of the mean bodies per system), theta and build. `lookup` returns the knobs
+    total and of the mean bodies per system), theta and build (walk and expansion; the
+    key width is one of the tuned knobs). `lookup` returns the knobs
     of a stored entry or None; `store` records `tune`'s result and rewrites the file."""
 
     def __init__(self, path: str):
@@ -201,14 +234,14 @@ class TuneCache:
         sizes = np.asarray(sizes, dtype=np.float64)
         total, mean = float(sizes.sum()), float(sizes.mean())
         mode = mode or {}
-        build = f"k{mode.get('morton_bits', 63)}w{mode.get('walk', lbvh.WALK_GROUP)}" \
-                f"q{mode.get('multipole', 2)}"
+        build = f"w{mode.get('walk', lbvh.WALK_GROUP)}q{mode.get('multipole', 2)}"
         return (f"N2^{int(math.floor(math.log2(max(total, 1.0))))}"
                 f"/n2^{int(math.floor(math.log2(max(mean, 1.0))))}/theta{theta:g}/{build}")
 
     def lookup(self, key: str):
         e = self.entries.get(key)
-        return None if e is None else {k: e[k] for k in KNOBS}
+        # entries written before the key width was tuned used 63-bit keys
+        return None if e is None else {k: e.get(k, 63) for k in KNOBS}
 
     def store(self, key: str, result: dict):
         self.entries[key] = {k: result[k] for k in KNOBS}

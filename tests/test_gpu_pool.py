@@ -208,6 +208,11 @@ def test_autotuner_block_shape_and_radix_width():
     say("tuned", {k: r[k] for k in ("group_size", "group_lanes", "radix_bits")},
         r["shape_table"], r["radix_table"])
     assert len(r["table"]) == 1 and len(r["shape_table"]) == 3 and len(r["radix_table"]) == 3
+    assert len(r["width_table"]) == 2 and r["morton_bits"] in (30, 63)
+    w = {row["morton_bits"]: row for row in r["width_table"]}
+    if r["morton_bits"] == 63:  # the exact build is given up only for speed or accuracy
+        assert (w[30]["median_err"] > 8e-4 or w[30]["p99_err"] > 1e-2 or
+                w[63]["rate"] > 1.02 * w[30]["rate"])
     ok = [t for t in r["shape_table"] if t["median_err"] <= 8e-4 and t["p99_err"] <= 1e-2]
     best = next(t for t in r["shape_table"]
                 if (t["group_size"], t["group_lanes"]) == (r["group_size"], r["group_lanes"]))
@@ -220,7 +225,8 @@ def test_autotuner_block_shape_and_radix_width():
         key = TuneCache.key([m.size for _, _, m in sy], 0.5)
         TuneCache(path).store(key, r)
         assert TuneCache(path).lookup(key) == {k: r[k] for k in (
-            "leaf_size", "far_theta", "mono_ratio", "group_size", "group_lanes", "radix_bits")}
+            "leaf_size", "far_theta", "mono_ratio", "morton_bits", "group_size", "group_lanes",
+            "radix_bits")}
 
 
 @pytest.mark.parametrize("mode", ["reference", "far"])

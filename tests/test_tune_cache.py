@@ -18,8 +18,8 @@ def test_keys_bucket_by_size_theta_and_build():
 
 def test_store_and_lookup_round_trip(tmp_path):
     path = str(tmp_path / "tune.json")
-    result = dict(leaf_size=1, far_theta=0.55, mono_ratio=0.8, group_size=64, group_lanes=4,
-                  radix_bits=9, table=[{"rate": 1.0}])
+    result = dict(leaf_size=1, far_theta=0.55, mono_ratio=0.8, morton_bits=30, group_size=64,
+                  group_lanes=4, radix_bits=9, table=[{"rate": 1.0}])
     c = TuneCache(path)
     assert c.lookup("absent") is None
     key = TuneCache.key([4096] * 8, 0.5)
@@ -28,3 +28,12 @@ def test_store_and_lookup_round_trip(tmp_path):
     assert got == {k: result[k] for k in KNOBS}
     with open(path) as f:
         assert set(json.load(f)[key]) == set(KNOBS)  # knobs only, no timing tables
+
+
+def test_entries_without_a_key_width_read_as_63_bits(tmp_path):
+    path = str(tmp_path / "old.json")
+    old = dict(leaf_size=1, far_theta=0.55, mono_ratio=0.8, group_size=64, group_lanes=4,
+               radix_bits=8)
+    with open(path, "w") as f:
+        json.dump({"k": old}, f)
+    assert TuneCache(path).lookup("k") == dict(old, morton_bits=63)
