@@ -409,115 +409,38 @@ def ic_timing(ic, n, seed, repeats=3):
             "device_ms": 1e3 * d, "host_over_device": host_s / d}
 
 
-def serve(glx, ic, Pool, seconds, rate, seed, budget=200 * 10**9, nmin=16384, nmax=2097152,
-          device_ic=False):
-    """Serving stream: collision jobs arrive as a Poisson process at `rate` per second
-    (N log-uniform in [nmin, nmax], theta uniform in {0.4, 0.5, 0.7}, steps uniform in
-    [50, 500], eps = 0.05, dt = 5e-3); a pool with a device-memory budget admits them in
-    arrival order as memory frees up and advances every active job by one KDK step per
-    iteration; finished jobs are retired. Initial conditions are made by a client thread
-    ahead of their arrival time (outside the server's clock when it keeps up), or, with
-    device_ic, drawn on the device at admission (glx_pool_admit_ic, inside the clock)."""
-    import queue
-    import threading
+SERVE_MODE = dict(morton_bits=63, walk=1, multipole=2, far_theta=0.5, mono_ratio=0.8)
 
-    rng = np.random.default_rng(seed)
-    njobs = int(rate * seconds * 1.5) + 8
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, njobs))
-    sizes = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), njobs))).astype(int)
-    thetas = rng.choice([0.4, 0.5, 0.7], njobs)
-    nsteps = rng.integers(50, 501, njobs)
-    ready: queue.Queue = queue.Queue(maxsize=8)
-    stop = threading.Event()
 
-    def client():
-        for k in range(njobs):
-            if stop.is_set():
-                return
-            pos, vel, mass, _ = ic.collision(int(sizes[k]), seed + 1000 + k)
-            while not stop.is_set():
-                try:
-                    ready.put((k, pos, vel, mass), timeout=0.2)
-                    break
-                except queue.Full:
-                    pass
+def serve(glx, Pool, seconds, rate, seed, warmup_s, step_budget_ms, budget=200 * 10**9,
+          device_ic=True, check_targets=64):
+    """Serving stream (galaxy_amd/serve.py): two-galaxy collision jobs arrive as a Poisson
+    process at `rate` per second (N log-uniform in [16,384, 2,097,152], theta uniform in
+    {0.4, 0.5, 0.7}, steps uniform in [50, 500], eps = 0.05, dt = 5e-3); one pool with a
+    device-memory budget admits them first come, first served as memory and the step-time
+    budget allow, advances every resident job one KDK step per pool step and retires
+    finished jobs. Initial conditions are drawn on the device at admission (inside the
+    server's clock). Latency and throughput are reported over the window after
+    `warmup_s` (the stream starts from an empty pool); each finished job's forces are
+    checked against the FP64 direct sum on `check_targets` seeded bodies."""
+    from galaxy_amd import ic
+    from galaxy_amd.serve import Server, poisson_collisions
 
-    th = threading.Thread(target=client, daemon=True)
-    pending = list(range(njobs)) if device_ic else []  # arrivals not yet in `waiting`
-    if not device_ic:
-        th.start()
-    pool = Pool(budget, max_jobs=4096, morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
-    waiting, active = [], {}
-    lat, pf, wall_steps = [], 0, 0.0
-    admitted = finished = rejected_wait = 0
-    queue_delay = []
-    t_start = time.perf_counter()
-    next_report = 30.0
-    while True:
-        now = time.perf_counter() - t_start
-        if now >= seconds:
-            break
-        if now >= next_report:  # progress of a long stream
-            log(f"serving {now:.0f}s: admitted {admitted}, finished {finished}, active "
-                f"{len(active)}, steps {len(lat)}")
-            next_report += 30.0
-        while not ready.empty():
-            waiting.append(ready.get())
-        while pending and arrivals[pending[0]] <= now:
-            waiting.append((pending.pop(0), None, None, None))
-        waiting.sort(key=lambda w: w[0])
-        still = []
-        for w in waiting:
-            k = w[0]
-            if arrivals[k] > now:
-                still.append(w)
-                continue
-            if device_ic:
-                jid = pool.admit_ic(ic.spec_collision(int(sizes[k]), seed + 1000 + k),
-                                    softening=0.05, theta=float(thetas[k]), dt=5e-3,
-                                    steps=int(nsteps[k]))
-            else:
-                jid = pool.admit(w[1], w[2], w[3], softening=0.05, theta=float(thetas[k]),
-                                 dt=5e-3, steps=int(nsteps[k]))
-            if jid is None:
-                rejected_wait += 1
-                still.append(w)
-                continue
-            active[jid] = int(sizes[k])
-            admitted += 1
-            queue_delay.append(time.perf_counter() - t_start - arrivals[k])
-        waiting = still
-        if not active:
-            time.sleep(0.001)
-            continue
-        t0 = time.perf_counter()
-        ms = pool.step()
-        wall_steps += time.perf_counter() - t0
-        lat.append(ms)
-        pf += sum(active.values())
-        for jid in list(active):
-            if pool.job(jid)["done"]:
-                pool.retire(jid)
-                del active[jid]
-                finished += 1
-    elapsed = time.perf_counter() - t_start
-    stop.set()
-    if th.is_alive():
-        th.join(timeout=30)
-    st = pool.status()
+    pool = Pool(budget, max_jobs=4096, **SERVE_MODE)
+    srv = Server(pool, step_budget_ms=step_budget_ms, direct_sum=glx.direct_sum_f64)
+    arrivals = poisson_collisions(rate, seconds, seed, device_ic=device_ic,
+                                  check_targets=check_targets,
+                                  make_state=lambda n, s: ic.collision(n, s)[:3])
+    srv.run(arrivals, seconds, progress=lambda v: log(
+        f"serving {v.now():.0f}s: {len(v.results)} done, {len(v.active)} active, "
+        f"{len(v.waiting)} waiting, {len(v.records)} steps"))
+    st = srv.stats(since=warmup_s)
+    st.update(seconds=seconds, warmup_s=warmup_s, rate_per_s=rate, budget_bytes=budget,
+              step_budget_ms=step_budget_ms, build=dict(SERVE_MODE),
+              initial_conditions="device, at admission" if device_ic else "host, at arrival",
+              evictions_total=pool.status()["evictions"])
     pool.close()
-    lat = np.array(lat) if lat else np.zeros(1)
-    return {
-        "seconds": elapsed, "rate_per_s": rate, "budget_bytes": budget,
-        "initial_conditions": "device, at admission" if device_ic else "host thread, ahead",
-        "jobs_admitted": admitted, "jobs_finished": finished, "jobs_active_at_end": st["active"],
-        "steps": int(lat.size),
-        "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
-        "particle_forces_per_s": pf / elapsed,
-        "particle_forces_per_s_stepping": pf / max(wall_steps, 1e-9),
-        "queue_delay_s": {"p50": float(np.percentile(queue_delay, 50)) if queue_delay else 0.0,
-                          "p99": float(np.percentile(queue_delay, 99)) if queue_delay else 0.0},
-    }
+    return st
 
 
 def strip(d):
@@ -541,9 +464,13 @@ def main():
     ap.add_argument("--precision-n", type=int, default=0,
                     help="also run the 1:3 collision precision stress at this N (0: skip)")
     ap.add_argument("--precision-steps", type=int, default=1000)
-    ap.add_argument("--serve-seconds", type=float, default=0.0,
-                    help="also run the serving stream for this long (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="job arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=90.0,
+                    help="serving stream length in seconds (0: skip; the workload names 600)")
+    ap.add_argument("--serve-warmup", type=float, default=30.0,
+                    help="seconds of the stream before the reported window")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
+    ap.add_argument("--serve-budget-ms", type=float, default=25.0,
+                    help="device-time budget of one serving step (0: none)")
     ap.add_argument("--leaf-size", type=int, default=0,
                     help="walk leaf size of the ensemble (0: autotune on 64 of its systems)")
     ap.add_argument("--far-theta", type=float, default=-1.0,
@@ -559,8 +486,8 @@ def main():
     ap.add_argument("--morton-bits", type=int, default=0, choices=(0, 30, 63),
                     help="Morton key width of the ensemble build, 30 or 63 (0: autotune)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
-    ap.add_argument("--serve-device-ic", action="store_true",
-                    help="serving jobs drawn on the device at admission (glx_pool_admit_ic)")
+    ap.add_argument("--serve-host-ic", action="store_true",
+                    help="serving jobs drawn on the host and uploaded (default: on the device)")
     ap.add_argument("--precision-only", action="store_true", help="only the precision stress")
     ap.add_argument("--profile", action="store_true",
                     help="pool steps only (for profilers): no plain, collision or CPU legs")
@@ -586,6 +513,7 @@ def main():
         return
     if args.profile:
         args.no_cpu = args.no_plain = args.no_collision = True
+        args.serve_seconds = 0.0
 
     import galaxy_amd as glx
     from galaxy_amd import ic
@@ -596,8 +524,9 @@ def main():
     if args.serve_only or args.precision_only:
         out = {}
         if args.serve_only:
-            out["serving"] = serve(glx, ic, Pool, args.serve_seconds or 60.0, args.serve_rate,
-                                   args.seed, device_ic=args.serve_device_ic)
+            out["serving"] = serve(glx, Pool, args.serve_seconds or 60.0, args.serve_rate,
+                                   args.seed, args.serve_warmup, args.serve_budget_ms,
+                                   device_ic=not args.serve_host_ic)
         if args.precision_only:
             out["precision_stress"] = precision(glx, ic, args.precision_n or (1 << 23), args.seed,
                                                 args.precision_steps, diag_every=100)
@@ -753,8 +682,9 @@ def main():
         out["collision_2p20"]["initial_conditions"] = ic_timing(ic, n, args.seed)
         log(f"collision ICs: {out['collision_2p20']['initial_conditions']}")
     if args.serve_seconds > 0:
-        out["serving"] = serve(glx, ic, Pool, args.serve_seconds, args.serve_rate, args.seed,
-                               device_ic=args.serve_device_ic)
+        out["serving"] = serve(glx, Pool, args.serve_seconds, args.serve_rate, args.seed,
+                               args.serve_warmup, args.serve_budget_ms,
+                               device_ic=not args.serve_host_ic)
         log(f"serving: {out['serving']}")
     if args.precision_n:
         out["precision_stress"] = precision(glx, ic, args.precision_n, args.seed,

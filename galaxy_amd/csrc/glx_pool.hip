@@ -3,6 +3,7 @@
 // segment-aware kernels. Admission is bounded by a device-memory budget and a slot
 // count; jobs join and leave between steps (continuous batching), and the step graph
 // is re-captured whenever the set of active jobs changes.
+#include <chrono>
 #include <map>
 #include <memory>
 
@@ -20,6 +21,7 @@ struct Job {
   uint64_t steps_done = 0;
   double time = 0.0;
   bool done = false;
+  bool evicted = false;  // stopped by the step budget
   bool primed = false;  // a(x0) computed (new jobs are primed together before their first step)
 };
 }  // namespace
@@ -38,6 +40,9 @@ struct glx_pool {
   uint64_t next_id = 1;
   uint64_t used = 0;
   uint64_t steps = 0;
+  uint64_t evictions = 0;
+  double step_budget_ms = 0.0;  // 0: no budget
+  double refresh_ms = 0.0;      // host time of the last step's refresh
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~glx_pool() {
     graph.drop();
@@ -53,7 +58,9 @@ struct glx_pool {
 // Rebuild the stepping batch from the active jobs (id order), prime the jobs admitted
 // since the last step with one batched force evaluation, and re-capture the step graph.
 static void pool_refresh(glx_pool* p) {
+  p->refresh_ms = 0.0;
   if (!p->dirty) return;
+  const auto t0 = std::chrono::steady_clock::now();
   p->graph.drop();
   p->active.clear();
   std::vector<SegDesc> descs, fresh;
@@ -64,7 +71,7 @@ static void pool_refresh(glx_pool* p) {
       fresh.push_back(j->seg.desc);
       unprimed.push_back(j);
     }
-    if (j->done) continue;
+    if (j->done || j->evicted) continue;
     p->active.push_back(j);
     descs.push_back(j->seg.desc);
   }
@@ -81,6 +88,20 @@ static void pool_refresh(glx_pool* p) {
     p->graph.capture(*p->batch);
   }
   p->dirty = false;
+  p->refresh_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// After a step over the budget: evict the most recently admitted active job.
+static void pool_enforce_budget(glx_pool* p, double ms) {
+  if (!(p->step_budget_ms > 0.0) || !(ms > p->step_budget_ms)) return;
+  Job* last = nullptr;  // jobs that finished with this step are not evicted
+  for (Job* j : p->active)
+    if (!j->done && (!last || j->id > last->id)) last = j;
+  if (!last) return;
+  last->evicted = true;
+  p->evictions += 1;
+  p->dirty = true;
 }
 
 static Job* find_job(glx_pool* p, uint64_t id) {
@@ -202,6 +223,15 @@ extern "C" int glx_pool_step(glx_pool* p, double* device_ms) {
     check(hipEventElapsedTime(&ms, p->ev0, p->ev1), "hipEventElapsedTime");
     if (device_ms) *device_ms = ms;
     pool_advance(p, 1);
+    pool_enforce_budget(p, ms);
+  });
+}
+
+extern "C" int glx_pool_set_step_budget(glx_pool* p, double budget_ms) {
+  return guard([&] {
+    if (!p) fail(GLX_EINVAL, "pool is NULL");
+    if (!(budget_ms >= 0.0)) fail(GLX_EINVAL, "step budget must be >= 0 ms");
+    p->step_budget_ms = budget_ms;
   });
 }
 
@@ -253,6 +283,7 @@ extern "C" int glx_pool_job(glx_pool* p, uint64_t id, glx_job_info* out) {
     out->steps = j->steps_target;
     out->time = j->time;
     out->done = j->done ? 1 : 0;
+    out->evicted = j->evicted ? 1 : 0;
   });
 }
 
@@ -286,6 +317,15 @@ extern "C" int glx_pool_snapshot_accel(glx_pool* p, uint64_t id, double* acc) {
   });
 }
 
+extern "C" int glx_pool_snapshot_mass(glx_pool* p, uint64_t id, double* mass) {
+  return guard([&] {
+    if (!p || !mass) fail(GLX_EINVAL, "NULL argument");
+    const Job* j = find_job(p, id);
+    d2h(mass, j->seg.desc.m, (size_t)j->seg.desc.n * sizeof(double), p->st);
+    sync(p->st);
+  });
+}
+
 extern "C" int glx_pool_retire(glx_pool* p, uint64_t id, double* pos, double* vel) {
   return guard([&] {
     if (!p) fail(GLX_EINVAL, "pool is NULL");
@@ -304,13 +344,16 @@ extern "C" int glx_pool_status(glx_pool* p, glx_pool_info* out) {
     *out = glx_pool_info{};
     out->jobs = (uint32_t)p->jobs.size();
     for (auto& kv : p->jobs)
-      if (!kv.second->done) {
+      if (!kv.second->done && !kv.second->evicted) {
         out->active += 1;
         out->particles += kv.second->seg.desc.n;
       }
     out->bytes_used = p->used;
     out->budget = p->budget;
     out->steps = p->steps;
+    out->evictions = p->evictions;
+    out->refresh_ms = p->refresh_ms;
+    out->step_budget_ms = p->step_budget_ms;
   });
 }
 

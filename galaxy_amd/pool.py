@@ -38,6 +38,7 @@ class JobInfo(C.Structure):
         ("steps", C.c_uint64),
         ("time", C.c_double),
         ("done", C.c_int),
+        ("evicted", C.c_int),
     ]
 
 
@@ -49,6 +50,9 @@ class PoolInfo(C.Structure):
         ("bytes_used", C.c_uint64),
         ("budget", C.c_uint64),
         ("steps", C.c_uint64),
+        ("evictions", C.c_uint64),
+        ("refresh_ms", C.c_double),
+        ("step_budget_ms", C.c_double),
     ]
 
 
@@ -63,11 +67,13 @@ def _bind():
         "glx_pool_admit": (I, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
         "glx_pool_admit_ic": (I, [P, P, D, D, D, C.c_uint64, C.POINTER(C.c_uint64)]),
         "glx_pool_step": (I, [P, C.POINTER(D)]),
+        "glx_pool_set_step_budget": (I, [P, D]),
         "glx_pool_step_timed": (I, [P, P]),
         "glx_pool_count": (I, [P, C.POINTER(Stats)]),
         "glx_pool_job": (I, [P, C.c_uint64, C.POINTER(JobInfo)]),
         "glx_pool_snapshot": (I, [P, C.c_uint64, P, P]),
         "glx_pool_snapshot_accel": (I, [P, C.c_uint64, P]),
+        "glx_pool_snapshot_mass": (I, [P, C.c_uint64, P]),
         "glx_pool_retire": (I, [P, C.c_uint64, P, P]),
         "glx_pool_status": (I, [P, C.POINTER(PoolInfo)]),
         "glx_pool_job_bytes": (C.c_uint64, [S]),
@@ -148,6 +154,11 @@ class Pool:
         _check(_bind().glx_pool_step(self._h, C.byref(ms)))
         return ms.value
 
+    def set_step_budget(self, ms: float) -> None:
+        """Device-time budget of one step (0: none): after a step over it the most
+        recently admitted active job is evicted (no longer advanced, kept until retired)."""
+        _check(_bind().glx_pool_set_step_budget(self._h, float(ms)))
+
     def step_timed(self) -> dict:
         ph = np.zeros(len(PHASES))
         _check(_bind().glx_pool_step_timed(self._h, _p(ph)))
@@ -162,7 +173,7 @@ class Pool:
         ji = JobInfo()
         _check(_bind().glx_pool_job(self._h, int(jid), C.byref(ji)))
         return {"id": ji.id, "n": ji.n, "steps_done": ji.steps_done, "steps": ji.steps,
-                "time": ji.time, "done": bool(ji.done)}
+                "time": ji.time, "done": bool(ji.done), "evicted": bool(ji.evicted)}
 
     def snapshot(self, jid: int):
         n = self._n[jid]
@@ -175,8 +186,17 @@ class Pool:
         _check(_bind().glx_pool_snapshot_accel(self._h, int(jid), _p(acc)))
         return acc
 
-    def retire(self, jid: int):
+    def snapshot_mass(self, jid: int) -> np.ndarray:
+        m = np.zeros(self._n[jid])
+        _check(_bind().glx_pool_snapshot_mass(self._h, int(jid), _p(m)))
+        return m
+
+    def retire(self, jid: int, copy: bool = True):
+        """Free the job; with copy, return its final (pos, vel) (else None)."""
         n = self._n.pop(jid)
+        if not copy:
+            _check(_bind().glx_pool_retire(self._h, int(jid), None, None))
+            return None
         pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
         _check(_bind().glx_pool_retire(self._h, int(jid), _p(pos), _p(vel)))
         return pos, vel
@@ -184,4 +204,5 @@ class Pool:
     def status(self) -> dict:
         pi = PoolInfo()
         _check(_bind().glx_pool_status(self._h, C.byref(pi)))
-        return {k: int(getattr(pi, k)) for k, _ in PoolInfo._fields_}
+        return {k: (float(getattr(pi, k)) if t is C.c_double else int(getattr(pi, k)))
+                for k, t in PoolInfo._fields_}

@@ -1,0 +1,164 @@
+"""CPU tests of the serving loop (galaxy_amd/serve.py) over a stand-in pool that keeps
+the glx_pool contract on the host: admission by memory and slots, one step advances every
+active job, jobs finish after their steps, and after a step over the step budget the most
+recently admitted active job is evicted (include/galaxy_lbvh.h, glx_pool_set_step_budget).
+The stand-in's step takes `cost` ms per weighted particle, so admission control, eviction
+and the reports can be checked without a GPU."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from galaxy_amd.serve import Request, Server, THETA_REF
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+
+class FakePool:
+    def __init__(self, memory, clock, cost=1e-3, slots=64, per_job_cost=None):
+        self.memory, self.clock, self.cost, self.slots = memory, clock, cost, slots
+        self.per_job_cost = per_job_cost or {}
+        self.jobs, self.order, self.budget, self.next_id, self.used = {}, [], 0.0, 1, 0
+        self.admitted = []
+
+    def set_step_budget(self, ms):
+        self.budget = ms
+
+    def _admit(self, n, theta, steps, tag):
+        if len(self.jobs) >= self.slots or self.used + n > self.memory:
+            return None
+        jid = self.next_id
+        self.next_id += 1
+        self.used += n
+        self.jobs[jid] = dict(n=n, theta=theta, steps=steps, steps_done=0, done=False,
+                              evicted=False, tag=tag)
+        self.admitted.append(tag)
+        return jid
+
+    def admit(self, pos, vel, mass, *, g, softening, theta, dt, steps):
+        return self._admit(mass.size, theta, steps, int(mass[0]))
+
+    def admit_ic(self, spec, *, softening, theta, dt, steps):
+        return self._admit(spec["n"], theta, steps, spec["tag"])
+
+    def step(self):
+        act = [j for j in self.jobs.values() if not j["done"] and not j["evicted"]]
+        ms = sum(self.per_job_cost.get(j["tag"], self.cost) * j["n"] *
+                 (THETA_REF / j["theta"]) ** 3 for j in act)
+        self.clock.t += ms * 1e-3
+        for j in act:
+            j["steps_done"] += 1
+            j["done"] = j["steps_done"] >= j["steps"]
+        if self.budget > 0 and ms > self.budget:
+            live = [k for k, j in self.jobs.items() if not j["done"] and not j["evicted"]]
+            if live:
+                self.jobs[max(live)]["evicted"] = True
+        return ms
+
+    def job(self, jid):
+        j = self.jobs[jid]
+        return {"id": jid, "n": j["n"], "steps_done": j["steps_done"], "steps": j["steps"],
+                "done": j["done"], "evicted": j["evicted"]}
+
+    def retire(self, jid, copy=False):
+        self.used -= self.jobs.pop(jid)["n"]
+        return None
+
+
+def req(tag, n, theta=0.5, steps=5):
+    mass = np.full(n, float(tag))  # the stand-in reads the tag from the first mass
+    return Request(n, theta, steps, state=(np.zeros((n, 3)), np.zeros((n, 3)), mass), tag=tag)
+
+
+def test_first_come_first_served_under_the_memory_budget():
+    clk = Clock()
+    pool = FakePool(memory=1000, clock=clk)
+    srv = Server(pool, clock=clk)
+    for tag, n in enumerate([600, 300, 500, 100, 200]):
+        srv.submit(req(tag, n, steps=3))
+    srv.drain()
+    # job 2 (500) waits for memory and the jobs behind it wait with it
+    assert pool.admitted == [0, 1, 2, 3, 4]
+    assert sorted(r.tag for r in srv.results) == [0, 1, 2, 3, 4]
+    assert all(r.steps_done == 3 and not r.evicted for r in srv.results)
+    assert srv.refused_memory >= 1
+    st = srv.stats()
+    assert st["jobs_finished"] == 5 and st["jobs_waiting"] == 0 and st["jobs_active"] == 0
+
+
+def test_admission_keeps_steps_within_the_budget():
+    clk = Clock()
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3)
+    srv = Server(pool, step_budget_ms=10.0, fill=0.85, clock=clk)
+    rng = np.random.default_rng(3)
+    for tag in range(60):
+        srv.submit(req(tag, int(rng.integers(200, 3000)), theta=float(rng.choice([0.4, 0.5, 0.7])),
+                   steps=int(rng.integers(3, 20))))
+    srv.drain()
+    assert len(srv.results) == 60 and not any(r.evicted for r in srv.results)
+    learned = [r for r in srv.records[2:]]  # the first steady step teaches the cost
+    assert max(r.device_ms for r in learned) <= 10.0
+    assert srv.cost == pytest.approx(1e-3)
+    # the admitted load stays near the budget while the queue is long
+    busy = [r.device_ms for r in srv.records[5:40]]
+    assert np.median(busy) > 0.5 * 10.0
+
+
+def test_eviction_is_reported_and_the_rest_continue():
+    clk = Clock()
+    # job 7 costs 20x its estimate: its first step overruns the budget and it is evicted
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3, per_job_cost={7: 2e-2})
+    srv = Server(pool, step_budget_ms=10.0, clock=clk)
+    for tag in range(6):
+        srv.submit(req(tag, 1000, steps=8))
+    srv.admit_ready()
+    for _ in range(3):
+        srv.step()
+    srv.submit(req(7, 1000, steps=8))
+    srv.drain()
+    by_tag = {r.tag: r for r in srv.results}
+    assert by_tag[7].evicted and by_tag[7].steps_done == 1
+    assert all(not by_tag[t].evicted and by_tag[t].steps_done == 8 for t in range(6))
+    assert srv.stats()["jobs_evicted"] == 1
+
+
+def test_a_job_over_the_budget_alone_is_refused_once_the_cost_is_known():
+    clk = Clock()
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3)
+    srv = Server(pool, step_budget_ms=5.0, clock=clk)
+    srv.submit(req(0, 1000, steps=4))
+    srv.admit_ready()
+    srv.step()
+    srv.step()  # a steady step: cost learned
+    assert srv.cost is not None
+    assert not srv.submit(req(1, 20000))  # 20 ms estimated alone
+    assert srv.submit(req(2, 2000))
+    srv.drain()
+    by_tag = {r.tag: r for r in srv.results}
+    assert by_tag[1].refused and by_tag[1].admitted is None
+    assert not by_tag[2].refused and by_tag[2].steps_done == 5
+    assert srv.stats()["jobs_refused"] == 1
+
+
+def test_run_submits_arrivals_at_their_times_and_reports_the_window():
+    clk = Clock()
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3)
+    def sleep(dt):
+        clk.t += dt
+
+    srv = Server(pool, clock=clk, sleep=sleep)
+    arrivals = [(0.0, req(0, 1000, steps=50)), (0.02, req(1, 500, steps=10)),
+                (0.5, req(2, 800, steps=10))]
+    srv.run(arrivals, seconds=1.0, idle_sleep=1e-3)
+    # arrival times relative to the server's start are honoured: job 2 waits for t=0.5
+    by_tag = {r.tag: r for r in srv.results}
+    assert by_tag[2].submitted >= 0.5
+    assert by_tag[1].submitted >= 0.02
+    st = srv.stats(since=0.0)
+    assert st["steps"] == len(srv.records) and st["particle_forces_per_s"] > 0
