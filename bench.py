@@ -339,12 +339,65 @@ def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets
     }
 
 
+PRECISION_FORMS = {
+    # FP32 walk in node/cube-relative coordinates with Kahan-compensated accumulation, FP64
+    # drift and FP64 diagnostics (the precision form)
+    "kahan": 1,
+    # the same walk with plain f32 accumulation
+    "f32": 0,
+    # the reference's plain f32 path: absolute f32 coordinates, f32 drift
+    "abs32": 2,
+}
+
+
+def _energy_run(glx, pos, vel, mass, eps, theta, dt, steps, prec, every, with_potential,
+                potential0=None):
+    """`steps` KDK steps of one precision form; FP64 diagnostics every `every` steps (the
+    exact FP64 potential too when with_potential; potential0, when given, is the initial
+    state's, already known). Returns the initial force array, the energy and momentum
+    traces and the step time."""
+    r = glx.GpuResidentLeapfrog(1.0, eps, theta, precision=prec, morton_bits=63,
+                                walk=glx.WALK_GROUP, multipole=2)
+    r.upload(pos, vel, mass)
+    acc0 = r.snapshot_accel()
+    d0 = r.diag(with_potential=with_potential and potential0 is None)
+    if potential0 is not None:
+        d0["potential"] = potential0
+    trace = [(0, d0["kinetic"], d0["potential"], d0["momentum"])]
+    t_steps, done = 0.0, 0
+    while done < steps:
+        k = min(every, steps - done)
+        t0 = time.perf_counter()
+        r.step_many(dt, k)  # returns when the steps are done
+        t_steps += time.perf_counter() - t0
+        done += k
+        d = r.diag(with_potential=with_potential)
+        trace.append((done, d["kinetic"], d["potential"], d["momentum"]))
+    r.close()
+    return acc0, trace, t_steps * 1e3 / max(steps, 1)
+
+
+def _drifts(trace, mass, vel):
+    """Energy drift max_t |E(t) - E(0)| / |E(0)| over the trace (E = kinetic + FP64
+    potential) and momentum drift max_t |P(t) - P(0)| / sum m |v|."""
+    e = np.array([k + u for _, k, u, _ in trace])
+    e0 = e[0]
+    p = np.array([m for _, _, _, m in trace])
+    psc = float(np.sum(mass * np.linalg.norm(vel, axis=1)))
+    return (float(np.max(np.abs(e - e0)) / abs(e0)),
+            float(np.max(np.linalg.norm(p - p[0], axis=1)) / psc))
+
+
 def precision(glx, ic, n, seed, steps, sample=65536, eps=0.02, theta=0.5, dt=2e-3,
-              diag_every=0):
-    """Precision stress: 1:3 two-galaxy collision. The FP32 walk with Kahan-compensated
-    accumulation against the all-FP32 walk (both with FP64 drift and FP64 diagnostics):
-    force error against an FP64 direct sum on `sample` seeded particles, then energy and
-    momentum drift over `steps` KDK steps (exact O(N^2) FP64 potential at the ends)."""
+              drift_n=1 << 18, drift_steps=1000, drift_every=10):
+    """Precision stress: the 1:3 two-galaxy collision (eps 0.02, theta 0.5, dt 2e-3).
+
+    At N = n: each form's force error against an FP64 direct sum on `sample` seeded
+    bodies, and its energy drift after `steps` KDK steps (exact FP64 potential at both
+    ends; an O(N^2) potential at N = 2^23 takes about 20 s, so the precision form and the
+    all-f32 path are run; the plain-f32 form's force error is reported too).
+    At N = drift_n: the energy drift as the maximum over steps, with the exact FP64
+    potential every `drift_every` steps over `drift_steps` steps, for all three forms."""
     pos, vel, mass, _ = ic.collision(n, seed, mass_ratio=3.0)
     tg = np.sort(np.random.default_rng(5).choice(n, min(n, sample), replace=False))
     tg = tg.astype(np.uint32)
@@ -352,35 +405,38 @@ def precision(glx, ic, n, seed, steps, sample=65536, eps=0.02, theta=0.5, dt=2e-
     d64 = glx.direct_sum_f64(pos, mass, 1.0, eps, tg)
     log(f"precision: fp64 direct sum on {tg.size} targets {time.perf_counter() - t0:.1f}s")
     res = {"n": n, "mass_ratio": 3.0, "eps": eps, "theta": theta, "dt": dt, "steps": steps,
-           "sample": int(tg.size)}
-    for name, prec in (("f32", glx.PREC_F32), ("kahan", glx.PREC_KAHAN)):
-        r = glx.GpuResidentLeapfrog(1.0, eps, theta, precision=prec, morton_bits=63,
-                                    walk=glx.WALK_GROUP, multipole=2)
-        r.upload(pos, vel, mass)
-        e = rel_err(r.snapshot_accel()[tg], d64)
-        d0 = r.diag(with_potential=True)
-        e0 = d0["kinetic"] + d0["potential"]
-        psc = float(np.sum(mass * np.linalg.norm(vel, axis=1)))
-        pdrift, trace = 0.0, []
+           "sample": int(tg.size), "build": "63-bit Morton, group walk, quadrupole",
+           "forms": {"kahan": "f32 walk relative to the root-cube centre, Kahan-compensated "
+                              "accumulation, f64 drift, f64 diagnostics",
+                     "f32": "f32 walk relative to the root-cube centre, plain accumulation, "
+                            "f64 drift",
+                     "abs32": "all-f32: absolute f32 coordinates, plain accumulation, f32 "
+                              "drift"}}
+    u0 = None  # FP64 potential of the initial state (the same for every form)
+    for name, prec in PRECISION_FORMS.items():
+        energy = name != "f32"
         t0 = time.perf_counter()
-        done = 0
-        while done < steps:
-            k = min(diag_every or steps, steps - done)
-            r.step_many(dt, k)
-            done += k
-            d = r.diag(with_potential=False)
-            pdrift = max(pdrift, float(np.linalg.norm(d["momentum"] - d0["momentum"])) / psc)
-            trace.append(done)
-        t_steps = time.perf_counter() - t0
-        d1 = r.diag(with_potential=True)
-        e1 = d1["kinetic"] + d1["potential"]
-        res[name] = {"force_error_vs_fp64_direct": quantiles(e),
-                     "energy_drift": abs(e1 - e0) / abs(e0),
-                     "momentum_drift_max": pdrift,
-                     "ms_per_step": t_steps * 1e3 / max(steps, 1),
-                     "energy": [e0, e1]}
-        log(f"precision {name}: {res[name]}")
-        r.close()
+        acc0, trace, ms = _energy_run(glx, pos, vel, mass, eps, theta, dt,
+                                      steps if energy else 0, prec, max(steps, 1), energy,
+                                      potential0=u0 if energy else None)
+        row = {"force_error_vs_fp64_direct": quantiles(rel_err(acc0[tg], d64))}
+        if energy:
+            u0 = trace[0][2]
+            de, dp = _drifts(trace, mass, vel)
+            row.update(energy_drift_end=de, momentum_drift=dp, ms_per_step=ms,
+                       energy=[trace[0][1] + trace[0][2], trace[-1][1] + trace[-1][2]])
+        res[name] = row
+        log(f"precision {name} ({time.perf_counter() - t0:.0f}s): {row}")
+    # the energy drift as a maximum over steps, at a size where the exact potential is cheap
+    pos, vel, mass, _ = ic.collision(drift_n, seed + 1, mass_ratio=3.0)
+    sub = {"n": drift_n, "steps": drift_steps, "every": drift_every}
+    for name, prec in PRECISION_FORMS.items():
+        _, trace, ms = _energy_run(glx, pos, vel, mass, eps, theta, dt, drift_steps, prec,
+                                   drift_every, True)
+        de, dp = _drifts(trace, mass, vel)
+        sub[name] = {"energy_drift_max": de, "momentum_drift_max": dp, "ms_per_step": ms}
+    res["drift_over_steps"] = sub
+    log(f"precision drift over steps: {sub}")
     return res
 
 
@@ -461,9 +517,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-collision", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--precision-n", type=int, default=0,
-                    help="also run the 1:3 collision precision stress at this N (0: skip)")
-    ap.add_argument("--precision-steps", type=int, default=1000)
+    ap.add_argument("--precision-n", type=int, default=1 << 23,
+                    help="N of the 1:3 collision precision stress (0: skip)")
+    ap.add_argument("--precision-steps", type=int, default=200,
+                    help="KDK steps of the precision stress at that N (the workload names 1000)")
+    ap.add_argument("--drift-n", type=int, default=1 << 18,
+                    help="N of the energy drift taken as a maximum over steps")
+    ap.add_argument("--drift-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=90.0,
                     help="serving stream length in seconds (0: skip; the workload names 600)")
     ap.add_argument("--serve-warmup", type=float, default=30.0,
@@ -514,6 +574,7 @@ def main():
     if args.profile:
         args.no_cpu = args.no_plain = args.no_collision = True
         args.serve_seconds = 0.0
+        args.precision_n = 0
 
     import galaxy_amd as glx
     from galaxy_amd import ic
@@ -529,7 +590,8 @@ def main():
                                    device_ic=not args.serve_host_ic)
         if args.precision_only:
             out["precision_stress"] = precision(glx, ic, args.precision_n or (1 << 23), args.seed,
-                                                args.precision_steps, diag_every=100)
+                                                args.precision_steps, drift_n=args.drift_n,
+                                                drift_steps=args.drift_steps)
         out["device"] = glx.device_info()
         print(json.dumps(out), flush=True)
         return
@@ -688,7 +750,8 @@ def main():
         log(f"serving: {out['serving']}")
     if args.precision_n:
         out["precision_stress"] = precision(glx, ic, args.precision_n, args.seed,
-                                            args.precision_steps, diag_every=100)
+                                            args.precision_steps, drift_n=args.drift_n,
+                                            drift_steps=args.drift_steps)
     out["device"] = glx.device_info()
     print(json.dumps(out), flush=True)
 

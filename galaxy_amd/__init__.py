@@ -16,6 +16,7 @@ from .lbvh import (  # noqa: F401
     lib,
     morton,
     sort,
+    PREC_ABS32,
     PREC_F32,
     PREC_KAHAN,
     NO_PARENT,

@@ -48,7 +48,7 @@ inline void validate_params(const glx_params* p) {
   if (p->leaf_size > 1 && p->walk != GLX_WALK_GROUP)
     fail(GLX_EINVAL, "leaf_size > 1 needs the group walk");
   if (p->multipole != 0 && p->multipole != 2) fail(GLX_EINVAL, "multipole must be 0 or 2");
-  if (p->precision > GLX_PREC_KAHAN) fail(GLX_EINVAL, "unknown precision mode");
+  if (p->precision > GLX_PREC_ABS32) fail(GLX_EINVAL, "unknown precision mode");
   if (p->morton_bits != 0 && p->morton_bits != 30 && p->morton_bits != 63)
     fail(GLX_EINVAL, "morton_bits must be 30 or 63");
   if (p->walk > GLX_WALK_GROUP) fail(GLX_EINVAL, "unknown walk");

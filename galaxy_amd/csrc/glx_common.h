@@ -57,7 +57,10 @@ __host__ __device__ constexpr int sort_start(bool key63, int radix_bits) {
   return sort_passes_for(key63, radix_bits) & 1;
 }
 
-enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1 };
+// walk accumulation and drift precision: kPrecF32 and kPrecKahan walk in f32 coordinates
+// relative to the root-cube centre and drift positions in f64; kPrecAbs32 is the plain
+// all-f32 path (f32 coordinates relative to the coordinate origin, f32 drift)
+enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1, kPrecAbs32 = 2 };
 
 // counters[] slots
 enum : int {

@@ -197,7 +197,8 @@ k_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_
   }
 }
 
-// Opening half-kick with a(x_n), f64 drift, and the bbox of the drifted positions
+// Opening half-kick with a(x_n), f64 drift (f32 on the all-f32 path), and the bbox of
+// the drifted positions
 // (core/src/integrator.rs:59-67; gpu/src/gpu_resident.rs:79-159).
 __global__ void __launch_bounds__(kBlock)
 k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
@@ -209,13 +210,16 @@ k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
     const SegDesc& s = segs[tl.seg];
     const float h = s.half_dt;
     const double dt = s.dt;
+    const bool abs32 = s.precision == kPrecAbs32;  // the all-f32 path drifts in f32
+    const float dt32 = (float)dt;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const size_t i = tl.start + e;
       for (int a = 0; a < 3; ++a) {
         const float v = s.v[3 * i + a] + s.acc[3 * i + a] * h;
         s.v[3 * i + a] = v;
-        const double x = s.x[3 * i + a] + (double)v * dt;
+        const double x = abs32 ? (double)((float)s.x[3 * i + a] + v * dt32)
+                               : s.x[3 * i + a] + (double)v * dt;
         s.x[3 * i + a] = x;
         lo[a] = fmin(lo[a], x);
         hi[a] = fmax(hi[a], x);
@@ -247,9 +251,11 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       s.bounds[2] = b.bmin[2];
       s.bounds[3] = b.size;
       s.bounds[4] = b.scale;
-      s.bounds[5] = b.origin[0];
-      s.bounds[6] = b.origin[1];
-      s.bounds[7] = b.origin[2];
+      // walk coordinates are relative to the cube centre, or absolute on the all-f32 path
+      const bool abs32 = s.precision == kPrecAbs32;
+      s.bounds[5] = abs32 ? 0.0 : b.origin[0];
+      s.bounds[6] = abs32 ? 0.0 : b.origin[1];
+      s.bounds[7] = abs32 ? 0.0 : b.origin[2];
     }
     K* keys = static_cast<K*>(s.keys[dst]);  // the sort's first source buffer
     uint32_t* vals = s.vals[dst];
@@ -281,7 +287,7 @@ __global__ void k_bounds(const SegDesc* __restrict__ segs, uint32_t nseg, int ce
   const Bounds b = seg_bounds(s.bbox, cell_bits);
   for (int a = 0; a < 3; ++a) {
     s.bounds[a] = b.bmin[a];
-    s.bounds[5 + a] = b.origin[a];
+    s.bounds[5 + a] = s.precision == kPrecAbs32 ? 0.0 : b.origin[a];
   }
   s.bounds[3] = b.size;
   s.bounds[4] = b.scale;
@@ -1597,6 +1603,19 @@ __global__ void k_diag_final(const SegDesc* __restrict__ segs, uint32_t nseg,
 
 constexpr int kDsTile = 256;
 
+// 1/sqrt(x) in f64: the f32 hardware estimate (relative error < 2^-22) refined by two
+// Newton steps y <- y (3/2 - x y^2 / 2), each of which squares the relative error, so the
+// result is within a few f64 ulps. Outside the range where x is a normal f32 it falls back
+// to the IEEE square root and division.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  if (__builtin_expect(!(x > 1e-36 && x < 1e36), 0)) return 1.0 / sqrt(x);
+  double y = (double)__builtin_amdgcn_rsqf((float)x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  y = y * fma(-hx * y, y, 1.5);
+  return y;
+}
+
 // Targets [blockIdx.x * 256, +256) against the sources of chunk blockIdx.y, [c * len,
 // min(n, (c + 1) * len)), self excluded. With one chunk the sums go straight to out
 // (3 per target); with several, chunk c writes partial[(c * nt + t) * 3 + axis] and
@@ -1627,8 +1646,8 @@ k_direct_f64(const double* __restrict__ x, const double* __restrict__ m, uint32_
     for (uint32_t q = 0; q < cnt; ++q) {
       const double dx = sx[q] - xi, dy = sy[q] - yi, dz = sz[q] - zi;
       const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-      const double inv_r3 = 1.0 / (r2 * sqrt(r2));
-      const double w = (base + q == i) ? 0.0 : g * inv_r3 * sm[q];
+      const double ir = rsqrt_f64(r2);
+      const double w = (base + q == i) ? 0.0 : sm[q] * (ir * ir * ir);
       ax += dx * w;
       ay += dy * w;
       az += dz * w;
@@ -1637,9 +1656,9 @@ k_direct_f64(const double* __restrict__ x, const double* __restrict__ m, uint32_
   }
   if (act) {
     double* o = out + ((size_t)blockIdx.y * nt + t) * 3;
-    o[0] = ax;
-    o[1] = ay;
-    o[2] = az;
+    o[0] = g * ax;
+    o[1] = g * ay;
+    o[2] = g * az;
   }
 }
 
@@ -1676,14 +1695,13 @@ k_potential_f64(const double* __restrict__ x, const double* __restrict__ m, uint
     __syncthreads();
     const uint32_t cnt = (n - base < (uint32_t)kDsTile) ? n - base : (uint32_t)kDsTile;
     for (uint32_t q = 0; q < cnt; ++q) {
-      if (base + q <= i) continue;
       const double dx = sx[q] - xi, dy = sy[q] - yi, dz = sz[q] - zi;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz + eps2);
-      u -= g * mi * sm[q] / r;
+      const double ir = rsqrt_f64(dx * dx + dy * dy + dz * dz + eps2);
+      u += (base + q > i) ? sm[q] * ir : 0.0;  // pairs j > i only (selected, not multiplied)
     }
     __syncthreads();
   }
-  if (!act) u = 0.0;
+  u = act ? -g * mi * u : 0.0;
   u = wave_sum(u);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = u;
   __syncthreads();

@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgalaxy_lbvh.so")
 
 GLX_OK, GLX_EINVAL, GLX_ENOMEM, GLX_EDEVICE, GLX_EBUDGET, GLX_ENOSLOT, GLX_ENOTFOUND = range(7)
-PREC_F32, PREC_KAHAN = 0, 1
+PREC_F32, PREC_KAHAN, PREC_ABS32 = 0, 1, 2
 FLAG_COUNT = 1
 WALK_REFERENCE, WALK_GROUP = 0, 1
 NO_PARENT = 0xFFFFFFFF

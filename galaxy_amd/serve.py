@@ -14,9 +14,10 @@ of the queue and the jobs behind it wait too, so a large job is never overtaken
 indefinitely. A job fits when the pool has a slot and the memory (`Pool.admit` returns
 None otherwise) and, with a step budget, when the estimated step time with it stays
 within `fill` x budget (before the first steady step has taught the cost, only into an
-empty pool). The estimate is a cost per weighted particle learned from the device time of
-the steps taken so far: a job of n particles at opening angle theta
-weighs n (0.5 / theta)^3 (the interaction count of a tree walk grows about as theta^-3).
+empty pool). The estimate is a fixed cost per step plus a cost per weighted particle,
+fitted to the device time of the recent steps whose active set did not change: a job of
+n particles at opening angle theta weighs n (0.5 / theta)^3 (the interaction count of a
+tree walk grows about as theta^-3).
 The pool evicts the most recently admitted job after a step that overran the budget
 (what the estimate missed); the server retires it and reports it as evicted. A job
 whose own estimated step exceeds the budget is refused when it is submitted.
@@ -116,6 +117,8 @@ class Server:
         self.results: list[Result] = []
         self.records: list[StepRecord] = []
         self.cost = None       # device ms per weighted particle (learned)
+        self.overhead = 0.0    # device ms of a step independent of its load (learned)
+        self._samples: deque = deque(maxlen=128)  # (weight, device ms) of steady steps
         self._changed = True
         self.refused_memory = 0  # admissions the pool turned down for memory or slots
 
@@ -131,7 +134,21 @@ class Server:
         return sum(e.req.n for e in self.active.values())
 
     def estimate_ms(self, weight: float) -> float | None:
-        return None if self.cost is None else self.cost * weight
+        return None if self.cost is None else self.overhead + self.cost * weight
+
+    def _learn(self, weight: float, ms: float) -> None:
+        """Step cost model ms = overhead + cost x weight, fitted by least squares to the
+        last 128 steady steps (proportional until their weights differ)."""
+        self._samples.append((weight, ms))
+        w = np.array([q[0] for q in self._samples])
+        t = np.array([q[1] for q in self._samples])
+        if len(w) >= 8 and np.ptp(w) > 0.1 * w.mean():
+            c = float(np.cov(w, t, bias=True)[0, 1] / np.var(w))
+            a = float(t.mean() - c * w.mean())
+            if c > 0 and a >= 0:
+                self.cost, self.overhead = c, a
+                return
+        self.cost, self.overhead = float(t.sum() / w.sum()), 0.0
 
     # ------------------------------------------------------------------ the loop
 
@@ -154,7 +171,7 @@ class Server:
             if self.budget > 0 and self.active:
                 # until a steady step has taught the cost, a job is admitted only into an
                 # empty pool
-                if self.cost is None or (self.cost * (self.active_weight() + e.req.weight()) >
+                if self.cost is None or (self.estimate_ms(self.active_weight() + e.req.weight()) >
                                          self.fill * self.budget):
                     break
             r = e.req
@@ -188,8 +205,7 @@ class Server:
         rec = StepRecord(t, wall, dev, len(self.active), particles, self._changed)
         self.records.append(rec)
         if not self._changed and weight > 0:  # steady steps teach the cost model
-            c = dev / weight
-            self.cost = c if self.cost is None else 0.8 * self.cost + 0.2 * c
+            self._learn(weight, dev)
         self._changed = False
         for jid in list(self.active):
             info = self.pool.job(jid)

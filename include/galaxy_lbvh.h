@@ -31,9 +31,13 @@ extern "C" {
 #define GLX_ENOSLOT 5   /* pool admission: no free job slot */
 #define GLX_ENOTFOUND 6 /* unknown job id */
 
-/* walk accumulation precision */
-#define GLX_PREC_F32 0   /* plain f32 accumulation */
-#define GLX_PREC_KAHAN 1 /* f32 with Kahan-compensated accumulation */
+/* walk and drift precision */
+#define GLX_PREC_F32 0   /* f32 walk relative to the root-cube centre, plain accumulation;
+                            f64 drift */
+#define GLX_PREC_KAHAN 1 /* the same with Kahan-compensated accumulation */
+#define GLX_PREC_ABS32 2 /* the reference's plain f32 path: f32 coordinates relative to
+                            the coordinate origin, plain accumulation, f32 drift
+                            (gpu/src/gpu_resident.rs:94-159 without double-single) */
 
 /* flags */
 #define GLX_FLAG_COUNT 1u /* count walk interactions / node visits (glx_solver_stats) */

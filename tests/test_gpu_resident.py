@@ -181,3 +181,44 @@ def test_kahan_resident_runs_and_tracks():
     r.step_many(DT, 30)
     s = r.snapshot()
     assert np.abs(s.pos - pr).max() < 1e-4 and np.abs(s.vel - vr).max() < 1e-4
+
+
+def test_all_f32_path_drifts_in_f32():
+    """The all-f32 form (GLX_PREC_ABS32) is the reference's plain f32 drift: bit-exact with
+    the sequential f32 sum x = fl32(x + fl32(v dt)), which loses the increments below a
+    coordinate's ulp (gpu/tests/gpu_resident.rs:378-411: ~3.5e-1 after 10^4 steps there),
+    while the FP64 drift of the other forms tracks the exact sum."""
+    x0 = np.array([[1024.0, -1024.0, 512.0]])
+    v0 = np.array([[1.0, -1.0, 0.5]])
+    r = resident(precision=glx.PREC_ABS32)
+    r.upload(x0, v0, np.ones(1))
+    r.step_many(DT, 10_000)
+    s = r.snapshot()
+    x = x0[0].astype(np.float32)
+    d = (v0[0].astype(np.float32) * np.float32(DT)).astype(np.float32)
+    for _ in range(10_000):
+        x = (x + d).astype(np.float32)
+    assert np.array_equal(s.pos[0], x.astype(np.float64))
+    exact = x0[0] + f32(v0)[0] * DT * 10_000
+    err = np.abs(s.pos[0] - exact).max()
+    say("all-f32 drift error after 1e4 steps", err)
+    assert err > 1e-2
+
+
+def test_all_f32_walk_loses_resolution_away_from_the_origin():
+    """A cluster centred 2000 units from the coordinate origin: the all-f32 walk (absolute
+    f32 coordinates) is farther from the FP64 direct sum than the walk in coordinates
+    relative to the root-cube centre, at theta -> 0 where the tree adds no error."""
+    pos, vel, mass = I.cluster(4, 2000, with_vel=True)
+    pos = pos + np.array([2000.0, -1500.0, 700.0])
+    tg = np.arange(0, 2000, 3, dtype=np.uint32)
+    d64 = glx.direct_sum_f64(pos, mass, G, EPS, tg)
+    errs = {}
+    for name, prec in (("rel", glx.PREC_F32), ("abs", glx.PREC_ABS32)):
+        r = glx.GpuResidentLeapfrog(G, EPS, 1e-6, prec)
+        r.upload(pos, vel, mass)
+        errs[name] = I.rel_err_quantiles(r.snapshot_accel()[tg], d64)
+        r.close()
+    say("theta0 errors, centre-relative vs absolute f32", errs)
+    assert errs["rel"][0] < 1e-5
+    assert errs["abs"][0] > 10 * errs["rel"][0]

@@ -21,8 +21,9 @@ class Clock:
 
 
 class FakePool:
-    def __init__(self, memory, clock, cost=1e-3, slots=64, per_job_cost=None):
+    def __init__(self, memory, clock, cost=1e-3, slots=64, per_job_cost=None, overhead=0.0):
         self.memory, self.clock, self.cost, self.slots = memory, clock, cost, slots
+        self.overhead = overhead
         self.per_job_cost = per_job_cost or {}
         self.jobs, self.order, self.budget, self.next_id, self.used = {}, [], 0.0, 1, 0
         self.admitted = []
@@ -50,7 +51,7 @@ class FakePool:
     def step(self):
         act = [j for j in self.jobs.values() if not j["done"] and not j["evicted"]]
         ms = sum(self.per_job_cost.get(j["tag"], self.cost) * j["n"] *
-                 (THETA_REF / j["theta"]) ** 3 for j in act)
+                 (THETA_REF / j["theta"]) ** 3 for j in act) + (self.overhead if act else 0.0)
         self.clock.t += ms * 1e-3
         for j in act:
             j["steps_done"] += 1
@@ -162,3 +163,18 @@ def test_run_submits_arrivals_at_their_times_and_reports_the_window():
     assert by_tag[1].submitted >= 0.02
     st = srv.stats(since=0.0)
     assert st["steps"] == len(srv.records) and st["particle_forces_per_s"] > 0
+
+
+def test_cost_model_learns_the_fixed_step_cost():
+    clk = Clock()
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3, overhead=2.0)
+    srv = Server(pool, step_budget_ms=12.0, fill=0.9, clock=clk)
+    rng = np.random.default_rng(8)
+    for tag in range(80):
+        srv.submit(req(tag, int(rng.integers(100, 4000)), steps=int(rng.integers(2, 15))))
+    srv.drain()
+    assert srv.overhead == pytest.approx(2.0) and srv.cost == pytest.approx(1e-3)
+    assert not any(r.evicted for r in srv.results)
+    # a job is refused when overhead + cost x weight alone exceeds the budget
+    assert srv.estimate_ms(9000) == pytest.approx(11.0)
+    assert srv.submit(req(90, 9000)) and not srv.submit(req(91, 11000))
