@@ -119,9 +119,9 @@ template <class T>
 struct DevArray {
   T* p = nullptr;
   explicit DevArray(size_t n) {
-    if (n) check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    if (n) dev_malloc(reinterpret_cast<void**>(&p), n * sizeof(T), "hipMalloc");
   }
-  ~DevArray() { (void)hipFree(p); }
+  ~DevArray() { dev_free(p); }
   DevArray(const DevArray&) = delete;
   DevArray& operator=(const DevArray&) = delete;
 };
@@ -233,6 +233,7 @@ struct StepGraph {
   }
   ~StepGraph() { drop(); }
   void capture(Batch& b) {
+    std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());
     drop();
     hipStream_t st = b.stream();
     check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "BeginCapture");

@@ -35,6 +35,7 @@ extern "C" int glx_device_malloc(size_t bytes, void** ptr) {
     *ptr = nullptr;
     ensure_device();
     if (bytes == 0) return;
+    std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());
     if (hipMalloc(ptr, bytes) != hipSuccess) {
       *ptr = nullptr;
       fail(GLX_ENOMEM, "hipMalloc failed");
@@ -44,14 +45,18 @@ extern "C" int glx_device_malloc(size_t bytes, void** ptr) {
 
 extern "C" int glx_device_free(void* ptr) {
   return guard([&] {
-    if (ptr) check(hipFree(ptr), "hipFree");
+    if (!ptr) return;
+    std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());
+    check(hipFree(ptr), "hipFree");
   });
 }
 
 extern "C" int glx_device_to_host(void* dst, const void* src, size_t bytes) {
   return guard([&] {
     if (bytes && (!dst || !src)) fail(GLX_EINVAL, "NULL pointer");
-    if (bytes) check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!bytes) return;
+    std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());  // a null-stream copy
+    check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
   });
 }
 
@@ -73,8 +78,8 @@ struct glx_solver {
   uint32_t cache_n = 0;     // bodies of the tree kept by the last build (0: none)
   ~glx_solver() {
     delete batch;
-    (void)hipFree(pot_buf);
-    (void)hipFree(mask);
+    dev_free(pot_buf);
+    dev_free(mask);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -149,10 +154,10 @@ static void solver_set_mask(glx_solver* s, size_t n, const size_t* active, size_
     m[active[q]] = 1;
   }
   if (s->mask_cap < n) {
-    (void)hipFree(s->mask);
+    dev_free(s->mask);
     s->mask = nullptr;
     s->mask_cap = 0;
-    check(hipMalloc(&s->mask, n), "hipMalloc");
+    dev_malloc(reinterpret_cast<void**>(&s->mask), n, "hipMalloc");
     s->mask_cap = n;
   }
   h2d(s->mask, m.data(), n, s->st);
@@ -274,10 +279,11 @@ extern "C" int glx_solver_potential_energy(glx_solver* s, size_t n, const double
     const size_t nb = (n + 255) / 256 + 1;
     const size_t need = 4 * n + nb;
     if (need > s->pot_cap) {
-      (void)hipFree(s->pot_buf);
+      dev_free(s->pot_buf);
       s->pot_buf = nullptr;
       s->pot_cap = 0;
-      check(hipMalloc(&s->pot_buf, need * sizeof(double)), "hipMalloc(potential)");
+      dev_malloc(reinterpret_cast<void**>(&s->pot_buf), need * sizeof(double),
+                 "hipMalloc(potential)");
       s->pot_cap = need;
     }
     double* px = s->pot_buf;

@@ -3,8 +3,9 @@
 // the device beyond the caller's host arrays (staged positions, masses and targets, the
 // output, the per-chunk partial sums) lives in one static device array of 64 MiB per
 // GPU, loaded with the code object and held until the process ends. Its device address
-// comes from hipGetSymbolAddress, once per device, on the first call on that device;
-// the array's name is never passed to a kernel or a copy. Calls on one device are
+// comes from hipGetSymbolAddress, once per device, on the first call on that device,
+// which also creates the device's stream for these calls; the array's name is never
+// passed to a kernel or a copy. Calls on one device are
 // serialised by a mutex (they share the array); the array keeps the last call's
 // contents and every byte a call reads is written by that call first.
 #include <mutex>
@@ -27,6 +28,7 @@ struct NoAllocDevice {
   int status = 0;
   std::string msg;
   unsigned char* ptr = nullptr;  // device address of `scratch` on this device
+  hipStream_t st = nullptr;      // this device's stream for the calls (non-blocking)
   std::mutex mu;
 };
 NoAllocDevice g_noalloc[kNoAllocDevices];
@@ -63,6 +65,13 @@ extern "C" int glx_direct_sum_f64_noalloc(size_t n, const double* pos, const dou
         return;
       }
       D.ptr = static_cast<unsigned char*>(p);
+      // a stream of its own: work on the legacy null stream would depend on (and could
+      // break) a graph capture in progress on another thread
+      const hipError_t es = hipStreamCreateWithFlags(&D.st, hipStreamNonBlocking);
+      if (es != hipSuccess) {
+        D.status = GLX_EDEVICE;
+        D.msg = std::string("hipStreamCreate: ") + hipGetErrorString(es);
+      }
     });
     if (D.status) fail(D.status, D.msg);
     std::lock_guard<std::mutex> lock(D.mu);
@@ -76,7 +85,7 @@ extern "C" int glx_direct_sum_f64_noalloc(size_t n, const double* pos, const dou
     double* out = reinterpret_cast<double*>(q);
     double* part = reinterpret_cast<double*>(D.ptr + kNoAllocStageBytes);
     const DirectPlan plan = direct_plan((uint32_t)n, (uint32_t)n_targets, kDirectPartialBytes);
-    hipStream_t st = nullptr;  // the null stream: no stream is created
+    hipStream_t st = D.st;
     h2d(x, pos, 3 * n * sizeof(double), st);
     h2d(m, mass, n * sizeof(double), st);
     h2d(tg, targets, n_targets * sizeof(uint32_t), st);

@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,14 @@ struct Error {
 [[noreturn]] void fail(int code, const std::string& msg);
 void check(hipError_t e, const char* what);
 void ensure_device();
+
+// Device allocation and graph capture exclude each other process-wide: a synchronous
+// runtime call (allocation, free, a null-stream memset) made in one thread while another
+// thread captures a step graph would join the capture or invalidate it. Every hipMalloc,
+// hipFree and null-stream memset of the library, and every capture, holds this lock.
+std::recursive_mutex& device_memory_mutex();
+void dev_malloc(void** p, size_t bytes, const char* what);  // throws GLX_ENOMEM / EDEVICE
+void dev_free(void* p);                                       // null is a no-op
 
 // One system's device arrays (a single allocation carved into the arrays of
 // SegDesc). Capacity grows on demand and never shrinks.

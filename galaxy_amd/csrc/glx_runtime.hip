@@ -15,6 +15,23 @@ void check(hipError_t e, const char* what) {
          std::string(what) + ": " + hipGetErrorString(e));
 }
 
+std::recursive_mutex& device_memory_mutex() {
+  static std::recursive_mutex mu;
+  return mu;
+}
+
+void dev_malloc(void** p, size_t bytes, const char* what) {
+  std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());
+  *p = nullptr;
+  check(hipMalloc(p, bytes), what);
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());
+  (void)hipFree(p);
+}
+
 void ensure_device() {
   static std::once_flag once;
   static int status = 0;
@@ -96,9 +113,7 @@ size_t Segment::bytes_for(uint32_t n, bool with_acc_out) {
   return layout_for(n, with_acc_out).total;
 }
 
-Segment::~Segment() {
-  if (block_) (void)hipFree(block_);
-}
+Segment::~Segment() { dev_free(block_); }
 
 void Segment::reserve(uint32_t n, bool with_acc_out) {
   if (block_ && n <= capacity && (has_acc_out || !with_acc_out)) return;
@@ -106,17 +121,18 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   if (block_ && n > capacity) cap = n > capacity + capacity / 2 ? n : capacity + capacity / 2;
   const bool acc_out = with_acc_out || has_acc_out;
   const Layout L = layout_for(cap, acc_out);
+  std::lock_guard<std::recursive_mutex> lock(device_memory_mutex());
   void* blk = nullptr;
-  check(hipMalloc(&blk, L.total), "hipMalloc(segment)");
+  dev_malloc(&blk, L.total, "hipMalloc(segment)");
   // arrival flags and counters must start at zero (kernels keep them zero afterwards)
   hipError_t e = hipMemset(static_cast<char*>(blk) + L.flag, 0, sizeof(uint32_t) * (cap ? cap : 1));
   if (e == hipSuccess)
     e = hipMemset(static_cast<char*>(blk) + L.counters, 0, kCounters * sizeof(unsigned long long));
   if (e != hipSuccess) {
-    (void)hipFree(blk);
+    dev_free(blk);
     check(e, "hipMemset(segment)");
   }
-  if (block_) (void)hipFree(block_);
+  dev_free(block_);
   block_ = blk;
   bytes_ = L.total;
   capacity = cap;
@@ -171,15 +187,15 @@ Batch::~Batch() {
   if (fork_) (void)hipEventDestroy(fork_);
   if (join_) (void)hipEventDestroy(join_);
   if (side_) (void)hipStreamDestroy(side_);
-  (void)hipFree(segs_);
-  (void)hipFree(etiles_);
-  (void)hipFree(atiles_);
-  (void)hipFree(wtiles_);
-  (void)hipFree(gtiles_);
-  (void)hipFree(tile_hist_);
-  (void)hipFree(tile_off_);
-  (void)hipFree(diag_partial_);
-  (void)hipFree(diag_out_);
+  dev_free(segs_);
+  dev_free(etiles_);
+  dev_free(atiles_);
+  dev_free(wtiles_);
+  dev_free(gtiles_);
+  dev_free(tile_hist_);
+  dev_free(tile_off_);
+  dev_free(diag_partial_);
+  dev_free(diag_out_);
 }
 
 template <class T>
@@ -188,8 +204,8 @@ static void grow(T*& p, size_t& cap, size_t need) {
   size_t c = need > cap * 2 ? need : cap * 2;
   if (c == 0) c = 1;
   T* q = nullptr;
-  check(hipMalloc(&q, c * sizeof(T)), "hipMalloc(batch table)");
-  (void)hipFree(p);
+  dev_malloc(reinterpret_cast<void**>(&q), c * sizeof(T), "hipMalloc(batch table)");
+  dev_free(p);
   p = q;
   cap = c;
 }
@@ -219,22 +235,26 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   size_t cs = cap_segs_;
   grow(segs_, cs, nseg_ ? nseg_ : 1);
   if (cs != cap_segs_ || !diag_out_) {
-    (void)hipFree(diag_out_);
+    dev_free(diag_out_);
     diag_out_ = nullptr;
-    check(hipMalloc(&diag_out_, cs * kDiagRecord * sizeof(double)), "hipMalloc(diag)");
+    dev_malloc(reinterpret_cast<void**>(&diag_out_), cs * kDiagRecord * sizeof(double),
+               "hipMalloc(diag)");
   }
   cap_segs_ = cs;
   size_t ce = cap_et_;
   grow(etiles_, ce, n_et_ ? n_et_ : 1);
   if (ce != cap_et_ || !tile_hist_) {
-    (void)hipFree(tile_hist_);
-    (void)hipFree(tile_off_);
-    (void)hipFree(diag_partial_);
+    dev_free(tile_hist_);
+    dev_free(tile_off_);
+    dev_free(diag_partial_);
     tile_hist_ = tile_off_ = nullptr;
     diag_partial_ = nullptr;
-    check(hipMalloc(&tile_hist_, ce * kRadixBinsMax * sizeof(uint32_t)), "hipMalloc(hist)");
-    check(hipMalloc(&tile_off_, ce * kRadixBinsMax * sizeof(uint32_t)), "hipMalloc(hist)");
-    check(hipMalloc(&diag_partial_, ce * kDiagRecord * sizeof(double)), "hipMalloc(diag)");
+    dev_malloc(reinterpret_cast<void**>(&tile_hist_), ce * kRadixBinsMax * sizeof(uint32_t),
+               "hipMalloc(hist)");
+    dev_malloc(reinterpret_cast<void**>(&tile_off_), ce * kRadixBinsMax * sizeof(uint32_t),
+               "hipMalloc(hist)");
+    dev_malloc(reinterpret_cast<void**>(&diag_partial_), ce * kDiagRecord * sizeof(double),
+               "hipMalloc(diag)");
   }
   cap_et_ = ce;
   grow(atiles_, cap_at_, n_at_ ? n_at_ : 1);
