@@ -485,9 +485,13 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
     c.batch.aggregate();
     std::vector<float4> lf(n), qd(quad && n > 1 ? 2 * (n - 1) : 0);
     std::vector<Node> nd(n > 1 ? n - 1 : 0);
+    std::vector<NodeBox> bx(n > 1 ? n - 1 : 0);
     double b[8];
     d2h(lf.data(), c.seg.desc.leaf, n * sizeof(float4), c.S.st);
-    if (n > 1) d2h(nd.data(), c.seg.desc.node, (n - 1) * sizeof(Node), c.S.st);
+    if (n > 1) {
+      d2h(nd.data(), c.seg.desc.node, (n - 1) * sizeof(Node), c.S.st);
+      d2h(bx.data(), c.seg.desc.box, (n - 1) * sizeof(NodeBox), c.S.st);
+    }
     if (!qd.empty()) d2h(qd.data(), c.seg.desc.quad, qd.size() * sizeof(float4), c.S.st);
     d2h(b, c.seg.desc.bounds, sizeof(b), c.S.st);
     sync(c.S.st);
@@ -502,8 +506,9 @@ extern "C" int glx_stage_aggregate(size_t n, const uint32_t* sorted_codes, const
     for (size_t j = 0; j + 1 < n; ++j) {
       const size_t u = n + j;
       const Node& q = nd[j];
-      const float cm[3] = {q.a.x, q.a.y, q.a.z}, mn[3] = {q.b.x, q.b.y, q.b.z},
-                  mx[3] = {q.c.x, q.c.y, q.c.z};
+      const NodeBox& w = bx[j];
+      const float cm[3] = {q.a.x, q.a.y, q.a.z}, mn[3] = {w.lo.x, w.lo.y, w.lo.z},
+                  mx[3] = {w.hi.x, w.hi.y, w.hi.z};
       for (int a = 0; a < 3; ++a) {
         com[3 * u + a] = cm[a];
         aabb_min[3 * u + a] = mn[a];

@@ -28,14 +28,22 @@ constexpr int kBlock = 256;                            // threads per block of e
 constexpr int kAggTile = 1024;                         // leaves per bottom-up tile (LDS pass)
 constexpr int kGroup = 64;                             // targets per group walk (one wave)
 
-// Walk record of one internal node (64 B).
+// Walk record of one internal node, split by use into two arrays of 32-B records: the
+// classifier record, which every node test of the group walk loads, and the box record,
+// which only the tests that need the node's box load (the per-target walk's inside test,
+// the group walk's disjoint-box test at theta > 0.55 and its bucket test). Siblings are
+// adjacent (children g and g + 1), so one 128-B line holds the classifier records of two
+// sibling pairs.
 struct __align__(16) Node {
   float4 a;  // centre of mass (origin-relative f32), mass
-  float4 b;  // AABB min (origin-relative f32), squared acceptance radius (s/theta + delta)^2
-  float4 c;  // AABB max (origin-relative f32), children: split | leaf flags (as bits)
-  uint4 d;   // next node after this subtree (unified or kEnd), right child, first leaf,
-             // last leaf; with quadrupoles the first two hold the group walk's squared
-             // monopole-band radius and the node's radius about its com (as bits)
+  float4 h;  // squared acceptance radius (s/theta + delta)^2, children: split | leaf flags
+             // (as bits); then, with quadrupoles, the group walk's squared monopole-band
+             // radius and the node's radius about its com, otherwise the next node after
+             // this subtree (unified or kEnd) and the right child (as bits)
+};
+struct __align__(16) NodeBox {
+  float4 lo;  // AABB min (origin-relative f32), first leaf (as bits)
+  float4 hi;  // AABB max (origin-relative f32), last leaf (as bits)
 };
 
 // Build/walk configuration shared by all segments of a launch sequence.
@@ -86,7 +94,8 @@ struct SegDesc {
   void* keys[2];         // sort keys: uint32_t (30-bit) or uint64_t (63-bit)
   uint32_t* vals[2];     // sort payload (original index)
   float4* leaf;          // n sorted leaves: origin-relative f32 position, mass
-  Node* node;            // n-1 internal-node walk records
+  Node* node;            // n-1 internal-node classifier records
+  NodeBox* box;          // n-1 internal-node box records
   float4* quad;          // 2(n-1) quadrupole moments {xx,yy,zz,xy},{xz,yz,-,-} or null
   uint32_t* left;        // n-1 left child (unified)
   uint32_t* right;       // n-1 right child (unified)

@@ -714,23 +714,25 @@ __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const A
   const float rcrit = size * inv_theta + delta;
   Node nd;
   nd.a = make_float4(r.com.x, r.com.y, r.com.z, r.mass);
-  nd.b = make_float4(r.mn.x, r.mn.y, r.mn.z, rcrit * rcrit);
-  nd.c = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(enc_children(cl, cr, s.n)));
+  const float ch = __uint_as_float(enc_children(cl, cr, s.n));
+  NodeBox bx;
+  bx.lo = make_float4(r.mn.x, r.mn.y, r.mn.z, __uint_as_float(lo));
+  bx.hi = make_float4(r.mx.x, r.mx.y, r.mx.z, __uint_as_float(hi));
   if (QUAD) {
     // the group walk's classifier radii: squared monopole-band radius (s / (theta
     // mono_ratio) + delta; FLT_MAX when the band is off) and the node's radius about its
     // com (far-field test); the skip pointer is rebuilt from rs[hi] where needed
-    const float bx = fmaxf(r.com.x - r.mn.x, r.mx.x - r.com.x),
-                by = fmaxf(r.com.y - r.mn.y, r.mx.y - r.com.y),
-                bz = fmaxf(r.com.z - r.mn.z, r.mx.z - r.com.z);
-    const float rn = sqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
+    const float ex = fmaxf(r.com.x - r.mn.x, r.mx.x - r.com.x),
+                ey = fmaxf(r.com.y - r.mn.y, r.mx.y - r.com.y),
+                ez = fmaxf(r.com.z - r.mn.z, r.mx.z - r.com.z);
+    const float rn = sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
     const float rm = fmaf(size, s.mono_inv, delta);
-    nd.d = make_uint4(__float_as_uint(s.mono_inv > 0.f ? rm * rm : 3.4e38f),
-                      __float_as_uint(rn), lo, hi);
+    nd.h = make_float4(rcrit * rcrit, ch, s.mono_inv > 0.f ? rm * rm : 3.4e38f, rn);
   } else {
-    nd.d = make_uint4(nx, cr, lo, hi);
+    nd.h = make_float4(rcrit * rcrit, ch, __uint_as_float(nx), __uint_as_float(cr));
   }
   s.node[j] = nd;
+  s.box[j] = bx;
   if (QUAD) {
     s.quad[2 * j] = make_float4(r.q[0], r.q[1], r.q[2], r.q[3]);
     s.quad[2 * j + 1] = make_float4(r.q[4], r.q[5], 0.f, 0.f);
@@ -749,9 +751,8 @@ template <bool QUAD>
 __device__ __forceinline__ AggV global_child(const SegDesc& s, uint32_t c) {
   const uint32_t n = s.n;
   if (c < n) return leaf_agg(s.leaf[c]);
-  const Node& nd = s.node[c - n];
   AggV r;
-  const float4 a = nd.a, b = nd.b, cc = nd.c;
+  const float4 a = s.node[c - n].a, b = s.box[c - n].lo, cc = s.box[c - n].hi;
   r.com = make_float3(a.x, a.y, a.z);
   r.mass = a.w;
   r.mn = make_float3(b.x, b.y, b.z);
@@ -957,19 +958,19 @@ k_flatten(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint
       }
       if (k + 1u < n) {
         const uint32_t q = flat_slot(s, n + k);
-        const Node nd = s.node[k];
+        const float4 na = s.node[k].a, nb = s.box[k].lo, nc = s.box[k].hi;
         const uint32_t lo = s.lo[k], hi = s.hi[k];
-        const float cx = (nd.b.x + nd.c.x) * 0.5f, cy = (nd.b.y + nd.c.y) * 0.5f,
-                    cz = (nd.b.z + nd.c.z) * 0.5f;
-        const float dx = nd.a.x - cx, dy = nd.a.y - cy, dz = nd.a.z - cz;
+        const float cx = (nb.x + nc.x) * 0.5f, cy = (nb.y + nc.y) * 0.5f,
+                    cz = (nb.z + nc.z) * 0.5f;
+        const float dx = na.x - cx, dy = na.y - cy, dz = na.z - cz;
         o.next[q] = q + 2u * (hi - lo + 1u) - 1u;  // 2 leaves - 1 nodes in the subtree
         o.body_start[q] = 0u;
         o.body_count[q] = 0u;
         flat_store3(o.center, q, cx, cy, cz);
-        flat_store3(o.half, q, (nd.c.x - nd.b.x) * 0.5f, (nd.c.y - nd.b.y) * 0.5f,
-                    (nd.c.z - nd.b.z) * 0.5f);
-        flat_store3(o.com, q, nd.a.x, nd.a.y, nd.a.z);
-        o.mass[q] = nd.a.w;
+        flat_store3(o.half, q, (nc.x - nb.x) * 0.5f, (nc.y - nb.y) * 0.5f,
+                    (nc.z - nb.z) * 0.5f);
+        flat_store3(o.com, q, na.x, na.y, na.z);
+        o.mass[q] = na.w;
         o.delta[q] = sqrtf(dx * dx + dy * dy + dz * dz);
       }
     }
@@ -1075,19 +1076,20 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
       u = nxt;
     } else {
       const Node* nd = node + (u - n);
-      const float4 A = nd->a;
-      // quadrupole builds keep classifier radii in d.xy: the skip pointer is rs[hi]
-      const uint32_t nxt = quad_records ? (nd->d.w == n - 1 ? kEnd : rs[nd->d.w]) : nd->d.x;
+      const float4 A = nd->a, H = nd->h;
+      const float4 B = s.box[u - n].lo, Cc = s.box[u - n].hi;
+      // quadrupole builds keep classifier radii in h.zw: the skip pointer is rs[hi]
+      const uint32_t hi = __float_as_uint(Cc.w);
+      const uint32_t nxt = quad_records ? (hi == n - 1 ? kEnd : rs[hi]) : __float_as_uint(H.z);
       if (!(A.w > 0.0f)) {
         u = nxt;
         continue;
       }
-      const float4 B = nd->b, Cc = nd->c;
       const bool inside = (B.x <= me.x) & (me.x <= Cc.x) & (B.y <= me.y) & (me.y <= Cc.y) &
                           (B.z <= me.z) & (me.z <= Cc.z);
       const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
       const float d2 = dx * dx + dy * dy + dz * dz;
-      if (!inside && d2 >= B.w) {
+      if (!inside && d2 >= H.x) {
         const float r2 = d2 + eps2;
         const float ir = rsqrtf(r2);
         const float w = A.w * ir * ir * ir;
@@ -1095,7 +1097,7 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
         if (COUNT) ++n_int;
         u = nxt;
       } else {
-        u = left_child(__float_as_uint(Cc.w), n);
+        u = left_child(__float_as_uint(H.y), n);
       }
     }
   }
@@ -1268,7 +1270,7 @@ __device__ __forceinline__ void far_drain(float* L, const SegDesc& s, const uint
                                           float cz, float eps2) {
   for (uint32_t e = lane; e < nf; e += kGroup) {
     const uint32_t j = fl[e];
-    const float4 A = gload4(reinterpret_cast<const float4*>(s.node), 4 * (size_t)j);
+    const float4 A = gload4(reinterpret_cast<const float4*>(s.node), 2 * (size_t)j);
     far_m2l(L, A.x - cx, A.y - cy, A.z - cz, A.w, gload4(s.quad, 2 * (size_t)j),
             gload4(s.quad, 2 * (size_t)j + 1), eps2);
   }
@@ -1316,6 +1318,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const float eps2 = s.eps2;
     const uint32_t bucket = s.leaf_size;
     const bool loose_theta = s.theta > 0.55f;  // the disjoint-box test is needed
+    const bool need_box = loose_theta || bucket > 1u;  // box records (bounds, leaf range)
     const uint64_t leaf_base = (uint64_t)s.leaf, node_base = (uint64_t)s.node;
     // the walk groups (subtrees of at most kGroup bodies, marked by the tree build)
     // that start in this chunk of sorted leaves, one after the other
@@ -1386,17 +1389,20 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const bool is_leaf = live && u < n;
         const bool is_node = live && u >= n;
         const uint32_t ni = is_node ? u - n : 0u;
-        const size_t nr4 = 4 * (size_t)(n > 1 ? ni : 0u);  // node record as four float4
+        const size_t nr2 = 2 * (size_t)(n > 1 ? ni : 0u);  // record as two float4
         const float4* nrec = reinterpret_cast<const float4*>(s.node);
         // a leaf's record (position, mass) has the layout of a node's first float4 (com,
         // mass), so one load serves both kinds
         const uint64_t abase = is_leaf ? leaf_base : node_base;
-        const uint32_t aidx = is_leaf ? u : 4u * (n > 1 ? ni : 0u);
+        const uint32_t aidx = is_leaf ? u : 2u * (n > 1 ? ni : 0u);
         const float4 A = gload4(reinterpret_cast<const float4*>(abase), aidx);
-        const float4 B = gload4(nrec, nr4 + 1), Cc = gload4(nrec, nr4 + 2);
-        const float4 Df = gload4(nrec, nr4 + 3);
-        const uint4 D = make_uint4(__float_as_uint(Df.x), __float_as_uint(Df.y),
-                                   __float_as_uint(Df.z), __float_as_uint(Df.w));
+        const float4 H = gload4(nrec, nr2 + 1);
+        float4 B = make_float4(0.f, 0.f, 0.f, 0.f), Cc = B;
+        if (need_box) {  // wave-uniform
+          const float4* brec = reinterpret_cast<const float4*>(s.box);
+          B = gload4(brec, nr2);
+          Cc = gload4(brec, nr2 + 1);
+        }
         // com relative to the group centre; distance from the group box to the com
         const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
         const float ex = fmaxf(fabsf(dx) - hx, 0.f);
@@ -1411,25 +1417,26 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
                                        : true;
         const bool massive = A.w > 0.0f;
         const bool node_live = is_node & massive;
-        const bool accept = node_live & apart & (dmin2 >= B.w);
-        const uint32_t nlo = D.z, ncount = D.w - D.z + 1u;
+        const bool accept = node_live & apart & (dmin2 >= H.x);
+        const uint32_t nlo = __float_as_uint(B.w), ncount = __float_as_uint(Cc.w) - nlo + 1u;
         const bool near = node_live & !accept;
-        const bool bkt = near & (ncount <= bucket);
+        // (an internal node holds at least two bodies: with buckets of one, never a bucket)
+        const bool bkt = near & (bucket > 1u) & (ncount <= bucket);
         const bool open = near & !bkt;
         // accepted and well separated from the whole group -> local expansion (node radius
         // about its com from the record)
         bool far = false;
         if (FAR) {
-          const float rn = __uint_as_float(D.y) + rg;
+          const float rn = H.w + rg;
           far = accept & (rn * rn <= s.far2 * fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
         }
         // an accepted node that also passes the opening test at theta * mono_ratio is a
         // monopole: the squared band radius is in the record
-        const bool mono = QUAD && accept && dmin2 >= __uint_as_float(D.x);
+        const bool mono = QUAD && accept && dmin2 >= H.z;
         const bool point = (is_leaf & massive) | (!QUAD & accept) | (mono & !far);
         const bool quadn = QUAD & accept & !point & !far;
-        const uint32_t cl = left_child(__float_as_uint(Cc.w), n),
-                       cr = right_child(__float_as_uint(Cc.w), n);
+        const uint32_t cl = left_child(__float_as_uint(H.y), n),
+                       cr = right_child(__float_as_uint(H.y), n);
         const unsigned long long m1 = __ballot(point), m2 = __ballot(quadn),
                                  m3 = __ballot(open), m4 = __ballot(bkt);
         const uint32_t c3 = (uint32_t)__popcll(m3);
@@ -1462,9 +1469,17 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         sp += 2u * c3;
         peak = sp > peak ? sp : peak;
         __syncthreads();
+        // The drains: with one lane per body (groups of more than 32) every lane takes
+        // every entry, so the loop runs on a wave-uniform index (scalar loop control,
+        // no per-lane bounds); lanes beyond the group compute terms they never store.
         if (np != 0u) {
+          if (rep == 1u) {
 #pragma unroll 2
-          for (uint32_t e = e0; e < np; e += rep) point_term<KAHAN>(a, me, pl[e], eps2);
+            for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
+          } else {
+#pragma unroll 2
+            for (uint32_t e = e0; e < np; e += rep) point_term<KAHAN>(a, me, pl[e], eps2);
+          }
           if (COUNT && e0 < np) n_int += (np - e0 + rep - 1u) / rep;
           np = 0;
         }
@@ -1476,9 +1491,14 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           nr = 0;
         }
         if (QUAD && nq != 0u) {
+          if (rep == 1u) {
 #pragma unroll 2
-          for (uint32_t e = e0; e < nq; e += rep)
-            quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
+            for (uint32_t e = 0; e < nq; ++e) quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
+          } else {
+#pragma unroll 2
+            for (uint32_t e = e0; e < nq; e += rep)
+              quad_term<KAHAN>(a, me, qa[e], qb[e], qc[e], eps2);
+          }
           if (COUNT && e0 < nq) n_quad += (nq - e0 + rep - 1u) / rep;
           nq = 0;
         }

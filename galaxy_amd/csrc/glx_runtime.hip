@@ -71,7 +71,7 @@ struct Carve {
 };
 
 struct Layout {
-  size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, quad, left, right, pleaf, pint, rs, lo,
+  size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, box, quad, left, right, pleaf, pint, rs, lo,
       hi, flag, roots, gend, bbox, bounds, counters, total;
 };
 
@@ -90,6 +90,7 @@ Layout layout_for(uint32_t n, bool with_acc_out) {
   L.v1 = c.take<uint32_t>(N);
   L.leaf = c.take<float4>(N);
   L.node = c.take<Node>(N);
+  L.box = c.take<NodeBox>(N);
   L.quad = c.take<float4>(2 * N);
   L.left = c.take<uint32_t>(N);
   L.right = c.take<uint32_t>(N);
@@ -150,6 +151,7 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   d.vals[1] = reinterpret_cast<uint32_t*>(b + L.v1);
   d.leaf = reinterpret_cast<float4*>(b + L.leaf);
   d.node = reinterpret_cast<Node*>(b + L.node);
+  d.box = reinterpret_cast<NodeBox*>(b + L.box);
   d.quad = reinterpret_cast<float4*>(b + L.quad);
   d.left = reinterpret_cast<uint32_t*>(b + L.left);
   d.right = reinterpret_cast<uint32_t*>(b + L.right);
