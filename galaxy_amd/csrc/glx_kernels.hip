@@ -1146,11 +1146,13 @@ constexpr int kFarCap = 128;  // far nodes are drained 64 at a time, one per lan
 constexpr float kR2Floor = 1e-20f;
 
 // Point-mass term of one source on the lane's target: a += m d / (|d|^2 + eps^2)^1.5.
-// The raw v_rsq (1 ulp) serves as rsqrt: r^2 is never denormal here.
-template <bool KAHAN>
+// The raw v_rsq (1 ulp) serves as rsqrt: r^2 is never denormal here. FLOOR applies the
+// r^2 floor; with eps^2 >= the floor, r^2 >= eps^2 already and the floor is left out.
+template <bool KAHAN, bool FLOOR = true>
 __device__ __forceinline__ void point_term(Acc3& a, const float4& me, const float4& p, float eps2) {
   const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
-  const float r2 = fmaxf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))), kR2Floor);
+  const float r2s = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+  const float r2 = FLOOR ? fmaxf(r2s, kR2Floor) : r2s;
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float w = p.w * (ir * ir) * ir;
   if (KAHAN) {
@@ -1316,6 +1318,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     if (tl.start == 0 && lane < 6) bbox_neutral(s.bbox, lane);
     const uint32_t n = s.n;
     const float eps2 = s.eps2;
+    const bool soft = eps2 >= kR2Floor;  // r^2 >= eps^2 needs no floor
     const uint32_t bucket = s.leaf_size;
     const bool loose_theta = s.theta > 0.55f;  // the disjoint-box test is needed
     const bool need_box = loose_theta || bucket > 1u;  // box records (bounds, leaf range)
@@ -1473,7 +1476,10 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         // every entry, so the loop runs on a wave-uniform index (scalar loop control,
         // no per-lane bounds); lanes beyond the group compute terms they never store.
         if (np != 0u) {
-          if (rep == 1u) {
+          if (rep == 1u && soft) {
+#pragma unroll 2
+            for (uint32_t e = 0; e < np; ++e) point_term<KAHAN, false>(a, me, pl[e], eps2);
+          } else if (rep == 1u) {
 #pragma unroll 2
             for (uint32_t e = 0; e < np; ++e) point_term<KAHAN>(a, me, pl[e], eps2);
           } else {
