@@ -210,19 +210,31 @@ k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
     const SegDesc& s = segs[tl.seg];
     const float h = s.half_dt;
     const double dt = s.dt;
-    const bool abs32 = s.precision == kPrecAbs32;  // the all-f32 path drifts in f32
-    const float dt32 = (float)dt;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
-      const size_t i = tl.start + e;
-      for (int a = 0; a < 3; ++a) {
-        const float v = s.v[3 * i + a] + s.acc[3 * i + a] * h;
-        s.v[3 * i + a] = v;
-        const double x = abs32 ? (double)((float)s.x[3 * i + a] + v * dt32)
-                               : s.x[3 * i + a] + (double)v * dt;
-        s.x[3 * i + a] = x;
-        lo[a] = fmin(lo[a], x);
-        hi[a] = fmax(hi[a], x);
+    if (s.precision == kPrecAbs32) {  // the all-f32 path drifts in f32 (tile-uniform)
+      const float dt32 = (float)dt;
+      for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
+        const size_t i = tl.start + e;
+        for (int a = 0; a < 3; ++a) {
+          const float v = s.v[3 * i + a] + s.acc[3 * i + a] * h;
+          s.v[3 * i + a] = v;
+          const double x = (double)((float)s.x[3 * i + a] + v * dt32);
+          s.x[3 * i + a] = x;
+          lo[a] = fmin(lo[a], x);
+          hi[a] = fmax(hi[a], x);
+        }
+      }
+    } else {
+      for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
+        const size_t i = tl.start + e;
+        for (int a = 0; a < 3; ++a) {
+          const float v = s.v[3 * i + a] + s.acc[3 * i + a] * h;
+          s.v[3 * i + a] = v;
+          const double x = s.x[3 * i + a] + (double)v * dt;
+          s.x[3 * i + a] = x;
+          lo[a] = fmin(lo[a], x);
+          hi[a] = fmax(hi[a], x);
+        }
       }
     }
     block_commit_bbox(lo, hi, s.bbox, sh);
