@@ -1412,11 +1412,21 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const uint32_t aidx = is_leaf ? u : 2u * (n > 1 ? ni : 0u);
         const float4 A = gload4(reinterpret_cast<const float4*>(abase), aidx);
         const float4 H = gload4(nrec, nr2 + 1);
-        float4 B = make_float4(0.f, 0.f, 0.f, 0.f), Cc = B;
-        if (need_box) {  // wave-uniform
+        // The box record only where it is needed (wave-uniform): the disjoint-box test
+        // (apart) and the bucket test's leaf range
+        bool apart = true;
+        uint32_t nlo = 0u, ncount = 2u;  // an internal node holds at least two bodies
+        if (need_box) {
           const float4* brec = reinterpret_cast<const float4*>(s.box);
-          B = gload4(brec, nr2);
-          Cc = gload4(brec, nr2 + 1);
+          const float4 B = gload4(brec, nr2), Cc = gload4(brec, nr2 + 1);
+          // Every point of the node's box lies within sqrt(3) s of its com, and an accepted
+          // com is at least s / theta + delta from the group box, so for theta <= 0.55
+          // (1 / theta > sqrt(3) with a margin) acceptance implies disjoint boxes.
+          if (loose_theta)
+            apart = (Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) | (B.y > gy1) | (Cc.z < gz0) |
+                    (B.z > gz1);
+          nlo = __float_as_uint(B.w);
+          ncount = __float_as_uint(Cc.w) - nlo + 1u;
         }
         // com relative to the group centre; distance from the group box to the com
         const float dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
@@ -1424,19 +1434,11 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         const float ey = fmaxf(fabsf(dy) - hy, 0.f);
         const float ez = fmaxf(fabsf(dz) - hz, 0.f);
         const float dmin2 = ex * ex + ey * ey + ez * ez;
-        // The boxes must be disjoint. Every point of the node's box lies within sqrt(3) s of
-        // its com, and an accepted com is at least s / theta + delta from the group box, so
-        // for theta <= 0.55 (1 / theta > sqrt(3) with a margin) acceptance implies it.
-        const bool apart = loose_theta ? ((Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) |
-                                          (B.y > gy1) | (Cc.z < gz0) | (B.z > gz1))
-                                       : true;
         const bool massive = A.w > 0.0f;
         const bool node_live = is_node & massive;
-        const bool accept = node_live & apart & (dmin2 >= H.x);
-        const uint32_t nlo = __float_as_uint(B.w), ncount = __float_as_uint(Cc.w) - nlo + 1u;
+        const bool accept = node_live & apart & (dmin2 >= H.x);  // boxes disjoint too
         const bool near = node_live & !accept;
-        // (an internal node holds at least two bodies: with buckets of one, never a bucket)
-        const bool bkt = near & (bucket > 1u) & (ncount <= bucket);
+        const bool bkt = near & (ncount <= bucket);
         const bool open = near & !bkt;
         // accepted and well separated from the whole group -> local expansion (node radius
         // about its com from the record)
