@@ -412,11 +412,16 @@ void stage_karras(size_t n, const K* sorted_codes, uint32_t* left, uint32_t* rig
   c.describe((uint32_t)n);
   h2d(c.seg.desc.keys[0], sorted_codes, n * sizeof(K), c.S.st);
   c.batch.karras();
-  d2h(left, c.seg.desc.left, (n - 1) * sizeof(uint32_t), c.S.st);
-  d2h(right, c.seg.desc.right, (n - 1) * sizeof(uint32_t), c.S.st);
+  std::vector<uint4> topo(n - 1);
+  d2h(topo.data(), c.seg.desc.topo, (n - 1) * sizeof(uint4), c.S.st);
   d2h(parent, c.seg.desc.parent_leaf, n * sizeof(uint32_t), c.S.st);
-  d2h(parent + n, c.seg.desc.parent_int, (n - 1) * sizeof(uint32_t), c.S.st);
   sync(c.S.st);
+  const uint32_t nn = (uint32_t)n;
+  for (size_t j = 0; j + 1 < n; ++j) {
+    left[j] = left_child(topo[j].z, nn);
+    right[j] = right_child(topo[j].z, nn);
+    parent[n + j] = topo[j].w;
+  }
 }
 }  // namespace
 

@@ -46,6 +46,20 @@ struct __align__(16) NodeBox {
   float4 hi;  // AABB max (origin-relative f32), last leaf (as bits)
 };
 
+// Children of an internal node in one word: the split position g (left child g, right
+// child g + 1, each a leaf or an internal node) and two leaf flags (n <= 2^29).
+constexpr uint32_t kLeftLeaf = 1u << 30, kRightLeaf = 1u << 31, kSplitMask = kLeftLeaf - 1u;
+__host__ __device__ inline uint32_t enc_children(uint32_t cl, uint32_t cr, uint32_t n) {
+  const uint32_t g = cl < n ? cl : cl - n;
+  return g | (cl < n ? kLeftLeaf : 0u) | (cr < n ? kRightLeaf : 0u);
+}
+__host__ __device__ inline uint32_t left_child(uint32_t e, uint32_t n) {
+  return (e & kLeftLeaf) ? (e & kSplitMask) : n + (e & kSplitMask);
+}
+__host__ __device__ inline uint32_t right_child(uint32_t e, uint32_t n) {
+  return (e & kRightLeaf) ? (e & kSplitMask) + 1u : n + (e & kSplitMask) + 1u;
+}
+
 // Build/walk configuration shared by all segments of a launch sequence.
 struct Mode {
   bool key63 = false;    // 63-bit Morton keys (2^21 cells per axis) instead of 30-bit
@@ -97,13 +111,11 @@ struct SegDesc {
   Node* node;            // n-1 internal-node classifier records
   NodeBox* box;          // n-1 internal-node box records
   float4* quad;          // 2(n-1) quadrupole moments {xx,yy,zz,xy},{xz,yz,-,-} or null
-  uint32_t* left;        // n-1 left child (unified)
-  uint32_t* right;       // n-1 right child (unified)
+  uint4* topo;           // n-1 internal node j: first leaf, last leaf of its range,
+                         // children (split | leaf flags, enc_children), parent (unified;
+                         // kNoParent for the root) — one load for the bottom-up climbs
   uint32_t* parent_leaf; // n parent of leaf k (unified)
-  uint32_t* parent_int;  // n-1 parent of internal node j (unified)
   uint32_t* rs;          // n-1 right child of the internal node whose split is s
-  uint32_t* lo;          // n-1 first leaf of internal node j's range
-  uint32_t* hi;          // n-1 last leaf of internal node j's range
   uint32_t* flag;        // n-1 arrival counters of the top bottom-up pass (kept at 0)
   uint32_t* roots;       // n local subtree roots handed from the tile pass to the top pass
   uint32_t* gend;        // n walk groups: last leaf of the group starting at k, else kEnd

@@ -595,14 +595,15 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       const uint32_t lu = (lo == gamma) ? (uint32_t)gamma : (uint32_t)(n + gamma);
       const uint32_t ru = (hi == gamma + 1) ? (uint32_t)(gamma + 1) : (uint32_t)(n + gamma + 1);
       const uint32_t me = (uint32_t)(n + i);
-      s.left[i] = lu;
-      s.right[i] = ru;
-      if (lu < (uint32_t)n) s.parent_leaf[lu] = me; else s.parent_int[lu - n] = me;
-      if (ru < (uint32_t)n) s.parent_leaf[ru] = me; else s.parent_int[ru - n] = me;
+      // node i's own fields; its parent field is written by its parent (the root's here)
+      uint32_t* tp = reinterpret_cast<uint32_t*>(s.topo);
+      tp[4 * (size_t)i] = (uint32_t)lo;
+      tp[4 * (size_t)i + 1] = (uint32_t)hi;
+      tp[4 * (size_t)i + 2] = enc_children(lu, ru, (uint32_t)n);
+      if (lu < (uint32_t)n) s.parent_leaf[lu] = me; else tp[4 * (size_t)(lu - n) + 3] = me;
+      if (ru < (uint32_t)n) s.parent_leaf[ru] = me; else tp[4 * (size_t)(ru - n) + 3] = me;
       s.rs[gamma] = ru;
-      s.lo[i] = (uint32_t)lo;
-      s.hi[i] = (uint32_t)hi;
-      if (i == 0) s.parent_int[0] = kNoParent;
+      if (i == 0) tp[3] = kNoParent;
       // walk groups: the subtrees of at most group_size (<= 64) bodies whose parent holds
       // more; a node above the limit marks those of its two children
       const int gmax = (int)s.group_size;
@@ -636,25 +637,16 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const 
   return (const __attribute__((address_space(1))) T*)p;
 }
 
-// Children of an internal node in one word: the split position g (left child g, right
-// child g + 1, each a leaf or an internal node) and two leaf flags (n <= 2^29).
-constexpr uint32_t kLeftLeaf = 1u << 30, kRightLeaf = 1u << 31, kSplitMask = kLeftLeaf - 1u;
-__device__ __forceinline__ uint32_t enc_children(uint32_t cl, uint32_t cr, uint32_t n) {
-  const uint32_t g = cl < n ? cl : cl - n;
-  return g | (cl < n ? kLeftLeaf : 0u) | (cr < n ? kRightLeaf : 0u);
-}
-__device__ __forceinline__ uint32_t left_child(uint32_t e, uint32_t n) {
-  return (e & kLeftLeaf) ? (e & kSplitMask) : n + (e & kSplitMask);
-}
-__device__ __forceinline__ uint32_t right_child(uint32_t e, uint32_t n) {
-  return (e & kRightLeaf) ? (e & kSplitMask) + 1u : n + (e & kSplitMask) + 1u;
-}
-
 // 16-byte record i of a float4 array as one global load (a flat load would also hold up
 // the LDS waits that follow it).
 __device__ __forceinline__ float4 gload4(const float4* p, size_t i) {
   const auto* q = gbl(reinterpret_cast<const float*>(p)) + 4 * i;
   return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+__device__ __forceinline__ uint4 gload_u4(const uint4* p, size_t i) {
+  const auto* q = gbl(reinterpret_cast<const uint32_t*>(p)) + 4 * i;
+  return make_uint4(q[0], q[1], q[2], q[3]);
 }
 
 struct AggV {
@@ -752,14 +744,6 @@ __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const A
 }
 
 template <bool QUAD>
-__device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const AggV& r,
-                                           float inv_theta) {
-  const uint32_t h = s.hi[j];
-  store_node<QUAD>(s, j, r, inv_theta, s.lo[j], h, s.left[j], s.right[j],
-                   h == s.n - 1 ? kEnd : s.rs[h]);
-}
-
-template <bool QUAD>
 __device__ __forceinline__ AggV global_child(const SegDesc& s, uint32_t c) {
   const uint32_t n = s.n;
   if (c < n) return leaf_agg(s.leaf[c]);
@@ -807,9 +791,10 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
       for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {
         const uint32_t j = p - n;
         if (j >= n - 1) break;
-        // all topology of node j in one batch of independent loads
-        const uint32_t lo = gbl(s.lo)[j], hi = gbl(s.hi)[j], cl = gbl(s.left)[j];
-        const uint32_t cr = gbl(s.right)[j], pp = gbl(s.parent_int)[j];
+        // all topology of node j in one 16-B load
+        const uint4 tj = gload_u4(s.topo, j);
+        const uint32_t lo = tj.x, hi = tj.y, cl = left_child(tj.z, n), cr = right_child(tj.z, n);
+        const uint32_t pp = tj.w;
         if (lo < t0 || hi >= t1) {
           const unsigned long long slot = atomicAdd(&s.counters[kCntRoots], 1ull);
           if (slot < n) s.roots[slot] = u;
@@ -882,7 +867,7 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
           const uint32_t r = atomicAdd(&next_root, 1u);
           if (r >= cnt) break;
           const uint32_t u = s.roots[r];
-          p = u < n ? s.parent_leaf[u] : s.parent_int[u - n];
+          p = u < n ? s.parent_leaf[u] : s.topo[u - n].w;
           continue;
         }
         const uint32_t j = p - n;
@@ -891,8 +876,9 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
           continue;
         }
         // topology comes from the tree build (an earlier kernel): load it with the arrival
-        const uint32_t lo = gbl(s.lo)[j], hi = gbl(s.hi)[j], cl = gbl(s.left)[j];
-        const uint32_t cr = gbl(s.right)[j], pp = gbl(s.parent_int)[j];
+        const uint4 tj = gload_u4(s.topo, j);
+        const uint32_t lo = tj.x, hi = tj.y, cl = left_child(tj.z, n), cr = right_child(tj.z, n);
+        const uint32_t pp = tj.w;
         const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0u) {
@@ -925,15 +911,16 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
 __device__ uint32_t flat_slot(const SegDesc& s, uint32_t u) {
   const uint32_t n = s.n;
   if (n == 1) return 0u;
-  const uint32_t lo = u < n ? u : gbl(s.lo)[u - n];
+  const uint32_t lo = u < n ? u : s.topo[u - n].x;
   uint32_t left_edges = 0, v = u;
-  uint32_t p = u < n ? gbl(s.parent_leaf)[u] : gbl(s.parent_int)[u - n];
+  uint32_t p = u < n ? gbl(s.parent_leaf)[u] : s.topo[u - n].w;
   for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {  // depth < n
     const uint32_t j = p - n;
     if (j >= n - 1) break;
-    if (gbl(s.left)[j] == v) ++left_edges;
+    const uint4 tj = s.topo[j];
+    if (left_child(tj.z, n) == v) ++left_edges;
     v = p;
-    p = gbl(s.parent_int)[j];
+    p = tj.w;
   }
   return 2u * lo + left_edges;
 }
@@ -971,7 +958,7 @@ k_flatten(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint
       if (k + 1u < n) {
         const uint32_t q = flat_slot(s, n + k);
         const float4 na = s.node[k].a, nb = s.box[k].lo, nc = s.box[k].hi;
-        const uint32_t lo = s.lo[k], hi = s.hi[k];
+        const uint32_t lo = s.topo[k].x, hi = s.topo[k].y;
         const float cx = (nb.x + nc.x) * 0.5f, cy = (nb.y + nc.y) * 0.5f,
                     cz = (nb.z + nc.z) * 0.5f;
         const float dx = na.x - cx, dy = na.y - cy, dz = na.z - cz;

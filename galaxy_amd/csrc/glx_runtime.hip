@@ -71,8 +71,8 @@ struct Carve {
 };
 
 struct Layout {
-  size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, box, quad, left, right, pleaf, pint, rs, lo,
-      hi, flag, roots, gend, bbox, bounds, counters, total;
+  size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, box, quad, topo, pleaf, rs, flag,
+      roots, gend, bbox, bounds, counters, total;
 };
 
 Layout layout_for(uint32_t n, bool with_acc_out) {
@@ -92,13 +92,9 @@ Layout layout_for(uint32_t n, bool with_acc_out) {
   L.node = c.take<Node>(N);
   L.box = c.take<NodeBox>(N);
   L.quad = c.take<float4>(2 * N);
-  L.left = c.take<uint32_t>(N);
-  L.right = c.take<uint32_t>(N);
+  L.topo = c.take<uint4>(N);
   L.pleaf = c.take<uint32_t>(N);
-  L.pint = c.take<uint32_t>(N);
   L.rs = c.take<uint32_t>(N);
-  L.lo = c.take<uint32_t>(N);
-  L.hi = c.take<uint32_t>(N);
   L.flag = c.take<uint32_t>(N);
   L.roots = c.take<uint32_t>(N);
   L.gend = c.take<uint32_t>(N);
@@ -153,13 +149,9 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   d.node = reinterpret_cast<Node*>(b + L.node);
   d.box = reinterpret_cast<NodeBox*>(b + L.box);
   d.quad = reinterpret_cast<float4*>(b + L.quad);
-  d.left = reinterpret_cast<uint32_t*>(b + L.left);
-  d.right = reinterpret_cast<uint32_t*>(b + L.right);
+  d.topo = reinterpret_cast<uint4*>(b + L.topo);
   d.parent_leaf = reinterpret_cast<uint32_t*>(b + L.pleaf);
-  d.parent_int = reinterpret_cast<uint32_t*>(b + L.pint);
   d.rs = reinterpret_cast<uint32_t*>(b + L.rs);
-  d.lo = reinterpret_cast<uint32_t*>(b + L.lo);
-  d.hi = reinterpret_cast<uint32_t*>(b + L.hi);
   d.flag = reinterpret_cast<uint32_t*>(b + L.flag);
   d.roots = reinterpret_cast<uint32_t*>(b + L.roots);
   d.gend = reinterpret_cast<uint32_t*>(b + L.gend);
