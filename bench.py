@@ -57,15 +57,18 @@ FLOPS_M2L = 150
 FLOPS_L2P = 60
 # HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
 # `bench.py --profile --leaf-size 1 --far-theta 0.55 --mono-ratio 0.8 --group-size 64
-# --group-lanes 4 --radix-bits 8` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled by the same
-# counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE reads half
-# the copied bytes (x2), WRITE_SIZE all of them (x1). 43.57 GB fetched + 0.56 GB written;
-# the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 43.573e9 + 0.558e9
+# --group-lanes 4 --radix-bits 8 --morton-bits 30` (--pmc FETCH_SIZE, then WRITE_SIZE),
+# scaled by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`):
+# FETCH_SIZE reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 31.28 GB fetched
+# + 0.56 GB written; the tested node records come from beyond L2.
+WALK_TRAFFIC_BYTES = 31.278e9 + 0.559e9
 # (leaf size, far_theta, mono_ratio, group size, lanes per body, key bits) the walk counters were
 # measured on
-WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4, 63)
-WALK_VALU_ISSUE = 0.88  # SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x walk time), same config
+WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4, 30)
+# SQ_INSTS_VALU / SQ_BUSY_CU_CYCLES of the walk, same config: VALU wave-instructions per
+# CU-cycle; one per CU-cycle (four SIMDs each taking four cycles per wave64 instruction) is
+# the issue limit, so the walk is VALU-issue bound at whatever clock the GPU runs
+WALK_VALU_PER_CU_CYCLE = 1.02
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 
 
@@ -237,13 +240,15 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
     cnt = pool.count()
     i_ref = (reference_interactions(glx, Pool, pool, ids, systems, eps, theta, dt)
              if with_ref else 0)
-    timed = [pool.step_timed() for _ in range(3)]
-    phases = {k: float(np.mean([t[k] for t in timed])) for k in timed[0]}
     dev = np.zeros(steps)
     t0 = time.perf_counter()
     for q in range(steps):
         dev[q] = pool.step()
     wall = time.perf_counter() - t0
+    # per-phase device times (HIP events between phases on the pool's stream), after the
+    # timed steps so that the clocks have settled as they had for them
+    timed = [pool.step_timed() for _ in range(5)]
+    phases = {k: float(np.mean([t[k] for t in timed])) for k in timed[0]}
     rng = np.random.default_rng(17)
     errs = []
     for j, (_, _, m) in zip(ids, systems):
@@ -696,14 +701,14 @@ def main():
             "traffic": WALK_TRAFFIC_BYTES if knobs == WALK_TRAFFIC_CONFIG else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
                               " leaf size 1, far-field ratio 0.55, monopole band 0.8, groups"
-                              " of 64, 4 lanes",
+                              " of 64, 4 lanes, 30-bit keys",
             "flops_per_launch": b["walk"]["flops_ref"],
             "executed_flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
-            # VALU wave-instructions issued per launch / what 256 CUs issue in its duration
-            # (rocprofv3 SQ_INSTS_VALU, profiles/sq_counters_walk_ensemble512.md): the
-            # walk is issue-bound although its counted flops are a small fraction of peak
-            "valu_issue_frac": WALK_VALU_ISSUE if knobs == WALK_TRAFFIC_CONFIG else None,
+            # VALU wave-instructions per CU busy cycle (rocprofv3 SQ_INSTS_VALU /
+            # SQ_BUSY_CU_CYCLES, profiles/sq_counters_walk_ensemble512.md): at the issue
+            # limit of one per cycle although its counted flops are a fraction of peak
+            "valu_per_cu_cycle": WALK_VALU_PER_CU_CYCLE if knobs == WALK_TRAFFIC_CONFIG else None,
         },
         "batched": strip(b),
         "tuning": tuning,
