@@ -683,6 +683,8 @@ def main():
         "p50_step_ms": b["device_ms"]["p50"],
         "p99_step_ms": b["device_ms"]["p99"],
         "force_error_vs_fp64_direct": b["error"],
+        # the metric's accuracy condition, checked on this run's forces
+        "median_force_error_within_1e-3": b["error"]["median"] <= 1e-3,
         "roofline": {
             "bound": "fp32-valu",
             "kernel": "k_walk_group",

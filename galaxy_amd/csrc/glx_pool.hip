@@ -21,7 +21,8 @@ struct Job {
   uint64_t steps_done = 0;
   double time = 0.0;
   bool done = false;
-  bool evicted = false;  // stopped by the step budget
+  bool evicted = false;  // paused by the step budget (until resumed or retired)
+  uint64_t activated = 0;  // order of admission or last resumption
   bool primed = false;  // a(x0) computed (new jobs are primed together before their first step)
 };
 }  // namespace
@@ -38,6 +39,7 @@ struct glx_pool {
   bool dirty = true;
   StepGraph graph;
   uint64_t next_id = 1;
+  uint64_t next_activation = 1;
   uint64_t used = 0;
   uint64_t steps = 0;
   uint64_t evictions = 0;
@@ -92,12 +94,13 @@ static void pool_refresh(glx_pool* p) {
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// After a step over the budget: evict the most recently admitted active job.
+// After a step over the budget: evict (pause) the most recently admitted or resumed
+// active job.
 static void pool_enforce_budget(glx_pool* p, double ms) {
   if (!(p->step_budget_ms > 0.0) || !(ms > p->step_budget_ms)) return;
   Job* last = nullptr;  // jobs that finished with this step are not evicted
   for (Job* j : p->active)
-    if (!j->done && (!last || j->id > last->id)) last = j;
+    if (!j->done && (!last || j->activated > last->activated)) last = j;
   if (!last) return;
   last->evicted = true;
   p->evictions += 1;
@@ -177,6 +180,7 @@ static void admit_job(glx_pool* p, size_t n, double g, double softening, double 
   load(j->seg.desc);
   sync(p->st);  // a(x0) follows at the next refresh, batched with the other new jobs
   j->id = p->next_id++;
+  j->activated = p->next_activation++;
   j->steps_target = steps;
   p->used += j->seg.bytes();
   *id = j->id;
@@ -224,6 +228,17 @@ extern "C" int glx_pool_step(glx_pool* p, double* device_ms) {
     if (device_ms) *device_ms = ms;
     pool_advance(p, 1);
     pool_enforce_budget(p, ms);
+  });
+}
+
+extern "C" int glx_pool_resume(glx_pool* p, uint64_t id) {
+  return guard([&] {
+    if (!p) fail(GLX_EINVAL, "pool is NULL");
+    Job* j = find_job(p, id);
+    if (!j->evicted) fail(GLX_EINVAL, "job is not evicted");
+    j->evicted = false;  // its cached a(x) is that of its current state: no priming
+    j->activated = p->next_activation++;
+    p->dirty = true;
   });
 }
 

@@ -68,6 +68,7 @@ def _bind():
         "glx_pool_admit_ic": (I, [P, P, D, D, D, C.c_uint64, C.POINTER(C.c_uint64)]),
         "glx_pool_step": (I, [P, C.POINTER(D)]),
         "glx_pool_set_step_budget": (I, [P, D]),
+        "glx_pool_resume": (I, [P, C.c_uint64]),
         "glx_pool_step_timed": (I, [P, P]),
         "glx_pool_count": (I, [P, C.POINTER(Stats)]),
         "glx_pool_job": (I, [P, C.c_uint64, C.POINTER(JobInfo)]),
@@ -156,8 +157,13 @@ class Pool:
 
     def set_step_budget(self, ms: float) -> None:
         """Device-time budget of one step (0: none): after a step over it the most
-        recently admitted active job is evicted (no longer advanced, kept until retired)."""
+        recently admitted or resumed active job is evicted (paused: not advanced, state
+        kept until `resume` or `retire`)."""
         _check(_bind().glx_pool_set_step_budget(self._h, float(ms)))
+
+    def resume(self, jid: int) -> None:
+        """Advance an evicted job again from its kept state."""
+        _check(_bind().glx_pool_resume(self._h, int(jid)))
 
     def step_timed(self) -> dict:
         ph = np.zeros(len(PHASES))

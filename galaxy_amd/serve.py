@@ -15,19 +15,26 @@ indefinitely. A job fits when the pool has a slot and the memory (`Pool.admit` r
 None otherwise) and, with a step budget, when the estimated step time with it stays
 within `fill` x budget (before the first steady step has taught the cost, only into an
 empty pool). The estimate is a fixed cost per step plus a cost per weighted particle,
-fitted to the device time of the recent steps whose active set did not change: a job of
-n particles at opening angle theta weighs n (0.5 / theta)^3 (the interaction count of a
-tree walk grows about as theta^-3).
-The pool evicts the most recently admitted job after a step that overran the budget
-(what the estimate missed); the server retires it and reports it as evicted. A job
-whose own estimated step exceeds the budget is refused when it is submitted.
+fitted to the device time of the recent steps whose active set did not change, plus
+the amount by which it underestimated the last step (the current regime): a job of
+n particles at opening angle theta weighs n (0.5 / theta)^3 log2(n) / 16 (the interaction
+count per particle of a tree walk grows about as theta^-3 log n).
+The pool evicts (pauses) the most recently admitted or resumed job after a step that
+overran the budget (what the estimate missed); the server charges the overrun to that job's weight, keeps
+its state on the device and resumes it, ahead of any new job, as soon as the estimate
+with it fits — a paused job is slowed down, not lost. A job whose own estimated step exceeds the budget is refused when
+it is submitted, and a job evicted while it ran alone is retired and reported as refused
+(it cannot meet the budget by itself).
 
 Step latency is recorded two ways per step: the host wall time of `Pool.step` (which
 includes the refresh after the active set changed: a(x0) of newly admitted jobs and the
-graph capture) and the device time of the step graph.
+graph capture) and the device time of the step graph. Percentiles are reported per pool
+step and per job step (each pool step counted once for every job it advanced: the
+latency the jobs see).
 """
 from __future__ import annotations
 
+import math
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -54,7 +61,8 @@ class Request:
     check_targets: int = 0  # FP64 force-error check on this many seeded bodies at the end
 
     def weight(self) -> float:
-        return float(self.n) * (THETA_REF / float(self.theta)) ** 3
+        return (float(self.n) * (THETA_REF / float(self.theta)) ** 3 *
+                math.log2(max(self.n, 2)) / 16.0)
 
 
 @dataclass
@@ -62,7 +70,7 @@ class Result:
     tag: object
     n: int
     steps_done: int
-    evicted: bool
+    preempted: int                    # times the step budget paused it
     refused: bool
     submitted: float
     admitted: float | None
@@ -79,6 +87,7 @@ class StepRecord:
     jobs: int
     particles: int
     changed: bool      # the active set changed since the previous step
+    estimate_ms: float  # the server's estimate of device_ms before the step (nan: none)
 
 
 @dataclass
@@ -87,11 +96,17 @@ class _Entry:
     submitted: float
     admitted: float | None = None
     jid: int | None = None
+    preempted: int = 0
+    scale: float = 1.0  # correction of the job's weight learned from an overrun
+
+    def weight(self) -> float:
+        return self.req.weight() * self.scale
 
 
 class Server:
     """Request loop over `pool` (a `galaxy_amd.pool.Pool`, or any object with its
-    admit / admit_ic / step / job / retire / snapshot_accel / snapshot methods).
+    admit / admit_ic / step / job / resume / retire / snapshot / snapshot_accel /
+    snapshot_mass methods).
 
     step_budget_ms: device-time budget of one step (0: none); fill: fraction of it the
     admission estimate may use; keep_state: keep each finished job's final (pos, vel);
@@ -113,12 +128,14 @@ class Server:
             pool.set_step_budget(self.budget)
         self.t0 = clock()
         self.waiting: deque[_Entry] = deque()
+        self.paused: list[_Entry] = []  # evicted by the budget, state on the device
         self.active: dict[int, _Entry] = {}
         self.results: list[Result] = []
         self.records: list[StepRecord] = []
         self.cost = None       # device ms per weighted particle (learned)
         self.overhead = 0.0    # device ms of a step independent of its load (learned)
         self._samples: deque = deque(maxlen=128)  # (weight, device ms) of steady steps
+        self._last_weight = 0.0  # weight of the last step
         self._changed = True
         self.refused_memory = 0  # admissions the pool turned down for memory or slots
 
@@ -128,13 +145,22 @@ class Server:
         return self.clock() - self.t0
 
     def active_weight(self) -> float:
-        return sum(e.req.weight() for e in self.active.values())
+        return sum(e.weight() for e in self.active.values())
 
     def active_particles(self) -> int:
         return sum(e.req.n for e in self.active.values())
 
     def estimate_ms(self, weight: float) -> float | None:
-        return None if self.cost is None else self.overhead + self.cost * weight
+        """Device time of a step over `weight`: the fitted model, plus the amount by which
+        the model underestimated the last step (the model is fitted over recent steps of
+        any load; the last step is the current regime)."""
+        if self.cost is None:
+            return None
+        est = self.overhead + self.cost * weight
+        if self.records and self._last_weight > 0:
+            miss = self.records[-1].device_ms - (self.overhead + self.cost * self._last_weight)
+            est += max(miss, 0.0)
+        return est
 
     def _learn(self, weight: float, ms: float) -> None:
         """Step cost model ms = overhead + cost x weight, fitted by least squares to the
@@ -158,22 +184,35 @@ class Server:
         t = self.now()
         est = self.estimate_ms(req.weight())
         if self.budget > 0 and est is not None and est > self.budget:
-            self.results.append(Result(req.tag, req.n, 0, False, True, t, None, t))
+            self.results.append(Result(req.tag, req.n, 0, 0, True, t, None, t))
             return False
         self.waiting.append(_Entry(req, t))
         return True
 
+    def _fits(self, e: _Entry) -> bool:
+        if self.budget <= 0 or not self.active:
+            return True
+        # until a steady step has taught the cost, a job is admitted only into an empty pool
+        return self.cost is not None and (self.estimate_ms(self.active_weight() + e.weight())
+                                          <= self.fill * self.budget)
+
     def admit_ready(self) -> int:
-        """Admit waiting jobs in arrival order while they fit; returns how many."""
+        """Resume paused jobs, then admit waiting jobs, in submission order while they
+        fit; returns how many joined."""
         admitted = 0
+        while self.paused:
+            e = self.paused[0]
+            if not self._fits(e):
+                return admitted
+            self.pool.resume(e.jid)
+            self.paused.pop(0)
+            self.active[e.jid] = e
+            self._changed = True
+            admitted += 1
         while self.waiting:
             e = self.waiting[0]
-            if self.budget > 0 and self.active:
-                # until a steady step has taught the cost, a job is admitted only into an
-                # empty pool
-                if self.cost is None or (self.estimate_ms(self.active_weight() + e.req.weight()) >
-                                         self.fill * self.budget):
-                    break
+            if not self._fits(e):
+                break
             r = e.req
             if r.spec is not None:
                 jid = self.pool.admit_ic(r.spec, softening=r.softening, theta=r.theta, dt=r.dt,
@@ -193,28 +232,46 @@ class Server:
         return admitted
 
     def step(self) -> StepRecord | None:
-        """One pool step of every resident job, then retire the finished and evicted
-        ones. None when nothing is resident."""
+        """One pool step of every resident job; then retire the finished ones and set
+        the evicted ones aside to resume. None when nothing is resident."""
         if not self.active:
             return None
         particles, weight = self.active_particles(), self.active_weight()
+        est = self.estimate_ms(weight)
         t = self.now()
         w0 = self.clock()
         dev = self.pool.step()
         wall = (self.clock() - w0) * 1e3
-        rec = StepRecord(t, wall, dev, len(self.active), particles, self._changed)
+        rec = StepRecord(t, wall, dev, len(self.active), particles, self._changed,
+                         est if est is not None else float("nan"))
         self.records.append(rec)
+        self._last_weight = weight
         if not self._changed and weight > 0:  # steady steps teach the cost model
             self._learn(weight, dev)
         self._changed = False
+        alone = len(self.active) == 1
         for jid in list(self.active):
             info = self.pool.job(jid)
-            if info["done"] or info["evicted"]:
+            if info["done"]:
                 self._finish(jid, info)
+            elif info["evicted"]:
+                e = self.active.pop(jid)
+                e.preempted += 1
+                self._changed = True
+                if est is not None and dev > est and self.cost:
+                    # the overrun is charged to the evicted job (the most recently joined,
+                    # whose cost is the least known): it resumes only when it fits with it
+                    e.scale *= 1.0 + (dev - est) / (self.cost * e.weight())
+                if alone:  # over the budget by itself: it cannot be served within it
+                    self._finish(jid, info, entry=e, refused=True)
+                else:
+                    self.paused.append(e)
+                    self.paused.sort(key=lambda q: q.submitted)
         return rec
 
-    def _finish(self, jid: int, info: dict) -> None:
-        e = self.active.pop(jid)
+    def _finish(self, jid: int, info: dict, entry: _Entry | None = None,
+                refused: bool = False) -> None:
+        e = entry if entry is not None else self.active.pop(jid)
         err = None
         if e.req.check_targets and self.direct_sum is not None and info["done"]:
             pos, _ = self.pool.snapshot(jid)
@@ -226,9 +283,9 @@ class Server:
             err = np.linalg.norm(acc[tg] - ref, axis=1) / np.linalg.norm(ref, axis=1)
         state = self.pool.retire(jid, copy=self.keep_state)
         self._changed = True
-        self.results.append(Result(e.req.tag, e.req.n, int(info["steps_done"]),
-                                   bool(info["evicted"]), False, e.submitted, e.admitted,
-                                   self.now(), state if self.keep_state else None, err))
+        self.results.append(Result(e.req.tag, e.req.n, int(info["steps_done"]), e.preempted,
+                                   refused, e.submitted, e.admitted, self.now(),
+                                   state if self.keep_state else None, err))
 
     def run(self, arrivals, seconds: float, idle_sleep: float = 1e-3, progress=None,
             progress_every: float = 30.0) -> None:
@@ -255,7 +312,7 @@ class Server:
         """Admit and step until every submitted job has finished."""
         for _ in range(max_steps):
             self.admit_ready()
-            if self.step() is None and not self.waiting:
+            if self.step() is None and not self.waiting and not self.paused:
                 return
         raise RuntimeError("drain: jobs still resident")
 
@@ -266,6 +323,8 @@ class Server:
         recs = [r for r in self.records if r.t >= since]
         wall = np.array([r.wall_ms for r in recs]) if recs else np.zeros(1)
         dev = np.array([r.device_ms for r in recs]) if recs else np.zeros(1)
+        jobs = np.array([r.jobs for r in recs]) if recs else np.ones(1, int)
+        wj = np.repeat(wall, jobs)  # one entry per job a step advanced
         span = max(self.now() - since, 1e-9)
         done = [r for r in self.results if not r.refused and r.finished >= since]
         delays = [r.admitted - r.submitted for r in self.results
@@ -277,13 +336,18 @@ class Server:
                              "p99": float(np.percentile(wall, 99)), "max": float(wall.max())},
             "step_ms_device": {"p50": float(np.percentile(dev, 50)),
                                "p99": float(np.percentile(dev, 99)), "max": float(dev.max())},
+            "job_step_ms_wall": {"p50": float(np.percentile(wj, 50)),
+                                 "p99": float(np.percentile(wj, 99))},
             "particle_forces_per_s": sum(r.particles for r in recs) / span,
             "particle_forces_per_s_stepping": sum(r.particles for r in recs) /
                                               max(sum(r.wall_ms for r in recs) * 1e-3, 1e-9),
-            "jobs_finished": sum(1 for r in done if not r.evicted),
-            "jobs_evicted": sum(1 for r in done if r.evicted),
+            "jobs_finished": len(done),
+            "jobs_preempted": sum(1 for r in done if r.preempted),
+            "preemptions": sum(r.preempted for r in done) + sum(e.preempted for e in
+                                                              self.paused + list(self.active.values())),
             "jobs_refused": sum(1 for r in self.results if r.refused),
-            "jobs_waiting": len(self.waiting), "jobs_active": len(self.active),
+            "jobs_waiting": len(self.waiting), "jobs_paused": len(self.paused),
+            "jobs_active": len(self.active),
             "queue_delay_s": {"p50": float(np.percentile(delays, 50)) if delays else 0.0,
                               "p99": float(np.percentile(delays, 99)) if delays else 0.0},
         }

@@ -306,8 +306,8 @@ typedef struct glx_job_info {
   uint64_t steps_done, steps;
   double time;
   int done;              /* steps_done reached steps: no longer advanced */
-  int evicted;           /* stopped by the pool's step budget (glx_pool_set_step_budget):
-                            no longer advanced; its state stays readable until retired */
+  int evicted;           /* paused by the pool's step budget (glx_pool_set_step_budget):
+                            not advanced until glx_pool_resume; state kept until retired */
 } glx_job_info;
 
 typedef struct glx_pool_info {
@@ -315,7 +315,7 @@ typedef struct glx_pool_info {
   uint64_t particles;    /* in active jobs */
   uint64_t bytes_used, budget;
   uint64_t steps;        /* pool steps taken */
-  uint64_t evictions;    /* jobs evicted by the step budget */
+  uint64_t evictions;    /* evictions by the step budget so far */
   double refresh_ms;     /* host time of the last step's refresh (a(x0) of newly admitted
                             jobs, segment tables and graph capture after the active set
                             changed; 0 when it did not change) */
@@ -338,14 +338,18 @@ int glx_pool_admit_ic(glx_pool *pool, const struct glx_ic_spec *spec, double sof
 /* One KDK step of every active job; device_ms (may be NULL) = device time of the step. */
 int glx_pool_step(glx_pool *pool, double *device_ms);
 /* Per-step time budget (device ms of one glx_pool_step; 0 = none, the default). After a
- * step that took longer, the most recently admitted active job is evicted: it is no longer
- * advanced, glx_pool_job reports evicted = 1 and it keeps its memory until retired. One
- * job per over-budget step, so the pool sheds load until its steps fit; a lone job that
- * does not fit is evicted too. The reference bounds device work per submit for the same
- * reason (MAX_BATCH, gpu/src/gpu_resident.rs:1271-1287); a server keeps steps within the
- * budget by admitting jobs by their estimated cost (galaxy_amd/serve.py), and eviction
- * is the guard for what the estimate misses. GLX_EINVAL for a negative or NaN budget. */
+ * step that took longer, the most recently admitted (or resumed) active job is evicted:
+ * paused — no longer advanced, glx_pool_job reports evicted = 1 — with its state and
+ * memory kept until glx_pool_resume or glx_pool_retire. One job per over-budget step, so
+ * the pool sheds load until its steps fit. The reference bounds device work per submit for
+ * the same reason (MAX_BATCH, gpu/src/gpu_resident.rs:1271-1287); a server keeps steps
+ * within the budget by admitting jobs by their estimated cost (galaxy_amd/serve.py), and
+ * eviction is the guard for what the estimate misses. GLX_EINVAL for a negative or NaN
+ * budget. */
 int glx_pool_set_step_budget(glx_pool *pool, double budget_ms);
+/* Resume an evicted job: it is advanced again from its kept state (its cached a(x) is
+ * that state's) with the next step. GLX_EINVAL if the job is not evicted. */
+int glx_pool_resume(glx_pool *pool, uint64_t id);
 /* One step with direct launches and per-phase device ms (see glx_sys_step_timed). */
 int glx_pool_step_timed(glx_pool *pool, double *phase_ms);
 /* Walk statistics summed over active jobs at their current positions. */

@@ -3,9 +3,10 @@ and the pool's step budget (glx_pool_set_step_budget).
 
 A served job's result is the same alone and in the mix (every kernel works per system):
 a job admitted mid-run, among jobs that arrive and leave around it, ends bit-identical to
-the same job stepped alone. Jobs wait for memory in arrival order; the step budget evicts
-the most recently admitted job and leaves the others' trajectories untouched; jobs drawn
-on the device carry an FP64 force-error check."""
+the same job stepped alone. Jobs wait for memory in arrival order; the step budget pauses
+the most recently admitted job and leaves the others' trajectories untouched, and a
+resumed job continues from its kept state; jobs drawn on the device carry an FP64
+force-error check."""
 from __future__ import annotations
 
 import numpy as np
@@ -63,7 +64,7 @@ def test_served_job_equals_the_same_job_alone():
     assert sorted(got) == [0, 1, 2, 3, 4]
     for r in reqs:
         res = got[r.tag]
-        assert res.steps_done == r.steps and not res.evicted
+        assert res.steps_done == r.steps and not res.preempted
         pos, vel = alone(r)
         assert np.array_equal(res.state[0], pos) and np.array_equal(res.state[1], vel), r.tag
 
@@ -85,10 +86,10 @@ def test_jobs_wait_for_memory_in_arrival_order():
     assert admitted == [0, 1, 2, 3, 4]
     assert srv.refused_memory >= 1
     assert sorted(r.tag for r in srv.results) == [0, 1, 2, 3, 4]
-    assert all(r.steps_done == 4 and not r.evicted for r in srv.results)
+    assert all(r.steps_done == 4 and not r.preempted for r in srv.results)
 
 
-def test_step_budget_evicts_the_newest_and_spares_the_rest():
+def test_step_budget_pauses_the_newest_and_spares_the_rest():
     reqs = [job(k, n, 0.5, 0, 20 + k) for k, n in enumerate([4000, 2500, 6000])]
     a, b = Pool(GB, **MODE), Pool(GB, **MODE)
     ia = [a.admit(*r.state, softening=0.05, theta=0.5, dt=5e-3) for r in reqs]
@@ -108,6 +109,16 @@ def test_step_budget_evicts_the_newest_and_spares_the_rest():
     pb, vb = b.snapshot(ib[0])
     assert np.array_equal(pa, pb) and np.array_equal(va, vb)
     assert a.step() == 0.0  # nothing left to step
+    # a resumed job continues from its kept state: job 2 paused after one step, resumed
+    # for two more, equals job 2 of the unbudgeted pool after three
+    a.set_step_budget(0.0)
+    a.resume(ia[2])
+    a.step()
+    a.step()
+    assert a.job(ia[2])["steps_done"] == 3 and not a.job(ia[2])["evicted"]
+    pa, va = a.snapshot(ia[2])
+    pb, vb = b.snapshot(ib[2])
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
     for j in ia:
         a.retire(j, copy=False)
     assert a.status()["bytes_used"] == 0

@@ -1,7 +1,8 @@
 """CPU tests of the serving loop (galaxy_amd/serve.py) over a stand-in pool that keeps
 the glx_pool contract on the host: admission by memory and slots, one step advances every
 active job, jobs finish after their steps, and after a step over the step budget the most
-recently admitted active job is evicted (include/galaxy_lbvh.h, glx_pool_set_step_budget).
+recently admitted or resumed active job is evicted — paused until resumed
+(include/galaxy_lbvh.h, glx_pool_set_step_budget, glx_pool_resume).
 The stand-in's step takes `cost` ms per weighted particle, so admission control, eviction
 and the reports can be checked without a GPU."""
 from __future__ import annotations
@@ -27,6 +28,7 @@ class FakePool:
         self.per_job_cost = per_job_cost or {}
         self.jobs, self.order, self.budget, self.next_id, self.used = {}, [], 0.0, 1, 0
         self.admitted = []
+        self.activation = 0
 
     def set_step_budget(self, ms):
         self.budget = ms
@@ -37,8 +39,9 @@ class FakePool:
         jid = self.next_id
         self.next_id += 1
         self.used += n
+        self.activation += 1
         self.jobs[jid] = dict(n=n, theta=theta, steps=steps, steps_done=0, done=False,
-                              evicted=False, tag=tag)
+                              evicted=False, tag=tag, act=self.activation)
         self.admitted.append(tag)
         return jid
 
@@ -50,8 +53,8 @@ class FakePool:
 
     def step(self):
         act = [j for j in self.jobs.values() if not j["done"] and not j["evicted"]]
-        ms = sum(self.per_job_cost.get(j["tag"], self.cost) * j["n"] *
-                 (THETA_REF / j["theta"]) ** 3 for j in act) + (self.overhead if act else 0.0)
+        ms = sum(self.per_job_cost.get(j["tag"], self.cost) * weight(j["n"], j["theta"])
+                 for j in act) + (self.overhead if act else 0.0)
         self.clock.t += ms * 1e-3
         for j in act:
             j["steps_done"] += 1
@@ -59,8 +62,14 @@ class FakePool:
         if self.budget > 0 and ms > self.budget:
             live = [k for k, j in self.jobs.items() if not j["done"] and not j["evicted"]]
             if live:
-                self.jobs[max(live)]["evicted"] = True
+                self.jobs[max(live, key=lambda k: self.jobs[k]["act"])]["evicted"] = True
         return ms
+
+    def resume(self, jid):
+        j = self.jobs[jid]
+        assert j["evicted"]
+        self.activation += 1
+        j["evicted"], j["act"] = False, self.activation
 
     def job(self, jid):
         j = self.jobs[jid]
@@ -70,6 +79,10 @@ class FakePool:
     def retire(self, jid, copy=False):
         self.used -= self.jobs.pop(jid)["n"]
         return None
+
+
+def weight(n, theta):
+    return n * (THETA_REF / theta) ** 3 * np.log2(max(n, 2)) / 16.0
 
 
 def req(tag, n, theta=0.5, steps=5):
@@ -87,7 +100,7 @@ def test_first_come_first_served_under_the_memory_budget():
     # job 2 (500) waits for memory and the jobs behind it wait with it
     assert pool.admitted == [0, 1, 2, 3, 4]
     assert sorted(r.tag for r in srv.results) == [0, 1, 2, 3, 4]
-    assert all(r.steps_done == 3 and not r.evicted for r in srv.results)
+    assert all(r.steps_done == 3 and not r.preempted for r in srv.results)
     assert srv.refused_memory >= 1
     st = srv.stats()
     assert st["jobs_finished"] == 5 and st["jobs_waiting"] == 0 and st["jobs_active"] == 0
@@ -102,7 +115,7 @@ def test_admission_keeps_steps_within_the_budget():
         srv.submit(req(tag, int(rng.integers(200, 3000)), theta=float(rng.choice([0.4, 0.5, 0.7])),
                    steps=int(rng.integers(3, 20))))
     srv.drain()
-    assert len(srv.results) == 60 and not any(r.evicted for r in srv.results)
+    assert len(srv.results) == 60 and not any(r.preempted for r in srv.results)
     learned = [r for r in srv.records[2:]]  # the first steady step teaches the cost
     assert max(r.device_ms for r in learned) <= 10.0
     assert srv.cost == pytest.approx(1e-3)
@@ -111,22 +124,40 @@ def test_admission_keeps_steps_within_the_budget():
     assert np.median(busy) > 0.5 * 10.0
 
 
-def test_eviction_is_reported_and_the_rest_continue():
+def test_an_evicted_job_is_paused_then_resumed_and_finishes():
     clk = Clock()
-    # job 7 costs 20x its estimate: its first step overruns the budget and it is evicted
+    # job 7 costs 20x its estimate: its first step overruns the budget, it is paused and
+    # resumes (ahead of later arrivals) once the others have left room
     pool = FakePool(memory=10**9, clock=clk, cost=1e-3, per_job_cost={7: 2e-2})
-    srv = Server(pool, step_budget_ms=10.0, clock=clk)
-    for tag in range(6):
+    srv = Server(pool, step_budget_ms=6.0, clock=clk)
+    for tag in range(4):
         srv.submit(req(tag, 1000, steps=8))
     srv.admit_ready()
     for _ in range(3):
         srv.step()
-    srv.submit(req(7, 1000, steps=8))
+    srv.submit(req(7, 400, steps=8))
+    srv.admit_ready()
+    assert 7 in [e.req.tag for e in srv.active.values()]
+    srv.step()
+    assert [e.req.tag for e in srv.paused] == [7]
+    srv.submit(req(8, 300, steps=2))  # arrives after 7 was paused: 7 goes first
     srv.drain()
     by_tag = {r.tag: r for r in srv.results}
-    assert by_tag[7].evicted and by_tag[7].steps_done == 1
-    assert all(not by_tag[t].evicted and by_tag[t].steps_done == 8 for t in range(6))
-    assert srv.stats()["jobs_evicted"] == 1
+    assert by_tag[7].preempted >= 1 and by_tag[7].steps_done == 8 and not by_tag[7].refused
+    assert all(by_tag[t].steps_done == 8 and not by_tag[t].preempted for t in range(4))
+    assert pool.admitted.index(8) > pool.admitted.index(7)
+    st = srv.stats()
+    assert st["jobs_preempted"] == 1 and st["jobs_finished"] == 6 and st["jobs_paused"] == 0
+
+
+def test_a_job_over_the_budget_by_itself_is_retired_as_refused():
+    clk = Clock()
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3, per_job_cost={3: 1.0})
+    srv = Server(pool, step_budget_ms=5.0, clock=clk)
+    srv.submit(req(3, 500, steps=10))
+    srv.drain()
+    r = srv.results[0]
+    assert r.refused and r.steps_done == 1 and r.preempted == 1
 
 
 def test_a_job_over_the_budget_alone_is_refused_once_the_cost_is_known():
@@ -174,7 +205,7 @@ def test_cost_model_learns_the_fixed_step_cost():
         srv.submit(req(tag, int(rng.integers(100, 4000)), steps=int(rng.integers(2, 15))))
     srv.drain()
     assert srv.overhead == pytest.approx(2.0) and srv.cost == pytest.approx(1e-3)
-    assert not any(r.evicted for r in srv.results)
+    assert not any(r.preempted for r in srv.results)
     # a job is refused when overhead + cost x weight alone exceeds the budget
-    assert srv.estimate_ms(9000) == pytest.approx(11.0)
-    assert srv.submit(req(90, 9000)) and not srv.submit(req(91, 11000))
+    assert srv.estimate_ms(weight(9000, 0.5)) == pytest.approx(2.0 + 1e-3 * weight(9000, 0.5))
+    assert srv.submit(req(90, 5000)) and not srv.submit(req(91, 13000))
