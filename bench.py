@@ -533,7 +533,7 @@ def main():
                     help="serving stream length in seconds (0: skip; the workload names 600)")
     ap.add_argument("--serve-warmup", type=float, default=30.0,
                     help="seconds of the stream before the reported window")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="job arrivals per second")
     ap.add_argument("--serve-budget-ms", type=float, default=25.0,
                     help="device-time budget of one serving step (0: none)")
     ap.add_argument("--leaf-size", type=int, default=0,

@@ -20,9 +20,10 @@ the amount by which it underestimated the last step (the current regime): a job 
 n particles at opening angle theta weighs n (0.5 / theta)^3 log2(n) / 16 (the interaction
 count per particle of a tree walk grows about as theta^-3 log n).
 The pool evicts (pauses) the most recently admitted or resumed job after a step that
-overran the budget (what the estimate missed); the server charges the overrun to that job's weight, keeps
-its state on the device and resumes it, ahead of any new job, as soon as the estimate
-with it fits — a paused job is slowed down, not lost. A job whose own estimated step exceeds the budget is refused when
+overran the budget (what the estimate missed); the server charges the overrun to that job (an extra time
+its resumption check adds), keeps its state on the device and resumes it, ahead of any new
+job, as soon as the estimate with it fits — a paused job is slowed down, not lost. While
+the last step took more than `fill` x budget, no job joins. A job whose own estimated step exceeds the budget is refused when
 it is submitted, and a job evicted while it ran alone is retired and reported as refused
 (it cannot meet the budget by itself).
 
@@ -97,10 +98,10 @@ class _Entry:
     admitted: float | None = None
     jid: int | None = None
     preempted: int = 0
-    scale: float = 1.0  # correction of the job's weight learned from an overrun
+    extra_ms: float = 0.0  # step time the model missed for this job (learned from overruns)
 
     def weight(self) -> float:
-        return self.req.weight() * self.scale
+        return self.req.weight()
 
 
 class Server:
@@ -135,7 +136,7 @@ class Server:
         self.cost = None       # device ms per weighted particle (learned)
         self.overhead = 0.0    # device ms of a step independent of its load (learned)
         self._samples: deque = deque(maxlen=128)  # (weight, device ms) of steady steps
-        self._last_weight = 0.0  # weight of the last step
+        self._last = None  # (weight, device ms) of the last step
         self._changed = True
         self.refused_memory = 0  # admissions the pool turned down for memory or slots
 
@@ -150,31 +151,40 @@ class Server:
     def active_particles(self) -> int:
         return sum(e.req.n for e in self.active.values())
 
-    def estimate_ms(self, weight: float) -> float | None:
-        """Device time of a step over `weight`: the fitted model, plus the amount by which
-        the model underestimated the last step (the model is fitted over recent steps of
+    def model_ms(self, weight: float) -> float | None:
+        """The fitted model's device time of a step over jobs of total `weight`."""
+        return None if self.cost is None else self.overhead + self.cost * weight
+
+    def estimate_ms(self, weight: float, extra: float = 0.0) -> float | None:
+        """Device time of a step over the active jobs plus more of total `weight` (the
+        active jobs' included) and learned overruns `extra`: the model, plus the amount by
+        which it underestimated the last step (the model is fitted over recent steps of
         any load; the last step is the current regime)."""
         if self.cost is None:
             return None
-        est = self.overhead + self.cost * weight
-        if self.records and self._last_weight > 0:
-            miss = self.records[-1].device_ms - (self.overhead + self.cost * self._last_weight)
-            est += max(miss, 0.0)
+        est = self.model_ms(weight) + extra
+        if self._last is not None:
+            lw, ld = self._last
+            est += max(ld - self.model_ms(lw), 0.0)
         return est
 
     def _learn(self, weight: float, ms: float) -> None:
         """Step cost model ms = overhead + cost x weight, fitted by least squares to the
-        last 128 steady steps (proportional until their weights differ)."""
+        last 128 steady steps. The fit is kept only when it is physical — a fixed cost no
+        larger than the cheapest step, a slope at least a quarter of the plain ratio of
+        time to weight — and otherwise (weights too alike, noise) the model is that
+        ratio alone."""
         self._samples.append((weight, ms))
         w = np.array([q[0] for q in self._samples])
         t = np.array([q[1] for q in self._samples])
+        ratio = float(t.sum() / w.sum())
         if len(w) >= 8 and np.ptp(w) > 0.1 * w.mean():
             c = float(np.cov(w, t, bias=True)[0, 1] / np.var(w))
             a = float(t.mean() - c * w.mean())
-            if c > 0 and a >= 0:
+            if c >= 0.25 * ratio and 0.0 <= a <= float(t.min()):
                 self.cost, self.overhead = c, a
                 return
-        self.cost, self.overhead = float(t.sum() / w.sum()), 0.0
+        self.cost, self.overhead = ratio, 0.0
 
     # ------------------------------------------------------------------ the loop
 
@@ -182,7 +192,7 @@ class Server:
         """Queue a job (arrival time = now). Returns False, and records a refused result,
         when its own estimated step exceeds the step budget."""
         t = self.now()
-        est = self.estimate_ms(req.weight())
+        est = self.model_ms(req.weight())  # the job alone
         if self.budget > 0 and est is not None and est > self.budget:
             self.results.append(Result(req.tag, req.n, 0, 0, True, t, None, t))
             return False
@@ -193,8 +203,12 @@ class Server:
         if self.budget <= 0 or not self.active:
             return True
         # until a steady step has taught the cost, a job is admitted only into an empty pool
-        return self.cost is not None and (self.estimate_ms(self.active_weight() + e.weight())
-                                          <= self.fill * self.budget)
+        # while the last step overran, no job joins; a paused job brings its learned
+        # overrun along
+        if self._last is not None and self._last[1] > self.fill * self.budget:
+            return False
+        est = self.estimate_ms(self.active_weight() + e.weight(), e.extra_ms)
+        return est is not None and est <= self.fill * self.budget
 
     def admit_ready(self) -> int:
         """Resume paused jobs, then admit waiting jobs, in submission order while they
@@ -245,9 +259,9 @@ class Server:
         rec = StepRecord(t, wall, dev, len(self.active), particles, self._changed,
                          est if est is not None else float("nan"))
         self.records.append(rec)
-        self._last_weight = weight
         if not self._changed and weight > 0:  # steady steps teach the cost model
             self._learn(weight, dev)
+        self._last = (weight, dev)
         self._changed = False
         alone = len(self.active) == 1
         for jid in list(self.active):
@@ -258,10 +272,10 @@ class Server:
                 e = self.active.pop(jid)
                 e.preempted += 1
                 self._changed = True
-                if est is not None and dev > est and self.cost:
+                if est is not None and dev > est:
                     # the overrun is charged to the evicted job (the most recently joined,
                     # whose cost is the least known): it resumes only when it fits with it
-                    e.scale *= 1.0 + (dev - est) / (self.cost * e.weight())
+                    e.extra_ms = max(e.extra_ms, dev - est)
                 if alone:  # over the budget by itself: it cannot be served within it
                     self._finish(jid, info, entry=e, refused=True)
                 else:

@@ -209,3 +209,34 @@ def test_cost_model_learns_the_fixed_step_cost():
     # a job is refused when overhead + cost x weight alone exceeds the budget
     assert srv.estimate_ms(weight(9000, 0.5)) == pytest.approx(2.0 + 1e-3 * weight(9000, 0.5))
     assert srv.submit(req(90, 5000)) and not srv.submit(req(91, 13000))
+
+
+def test_an_unphysical_fit_falls_back_to_the_ratio():
+    """Steps whose times do not grow with their weight (noise around a level) must not
+    produce a near-zero slope that would admit everything."""
+    clk = Clock()
+    srv = Server(FakePool(memory=10**9, clock=clk), step_budget_ms=25.0, clock=clk)
+    rng = np.random.default_rng(1)
+    for k in range(64):
+        w = float(rng.uniform(1e6, 3e6))
+        srv._learn(w, 20.0 + float(rng.normal(0, 0.5)))
+    assert srv.overhead == 0.0
+    assert srv.cost == pytest.approx(20.0 / 2e6, rel=0.1)
+
+
+def test_heavier_than_estimated_jobs_do_not_derail_the_model():
+    """Every seventh job costs 4x its weight: its overruns are charged to it (it is paused
+    and resumed), while the model keeps the typical cost and the steps stay near the
+    budget."""
+    clk = Clock()
+    heavy = {t: 4e-3 for t in range(0, 200, 7)}
+    pool = FakePool(memory=10**9, clock=clk, cost=1e-3, per_job_cost=heavy)
+    srv = Server(pool, step_budget_ms=10.0, clock=clk)
+    rng = np.random.default_rng(4)
+    for tag in range(200):
+        srv.submit(req(tag, int(rng.integers(200, 2500)), steps=int(rng.integers(3, 30))))
+    srv.drain()
+    assert len(srv.results) == 200 and not any(r.refused for r in srv.results)
+    assert 0.5e-3 < srv.cost < 2e-3
+    dev = np.array([r.device_ms for r in srv.records])
+    assert np.mean(dev > 10.0) < 0.05 and dev.max() < 20.0
