@@ -1076,19 +1076,27 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
     } else {
       const Node* nd = node + (u - n);
       const float4 A = nd->a, H = nd->h;
-      const float4 B = s.box[u - n].lo, Cc = s.box[u - n].hi;
-      // quadrupole builds keep classifier radii in h.zw: the skip pointer is rs[hi]
-      const uint32_t hi = __float_as_uint(Cc.w);
-      const uint32_t nxt = quad_records ? (hi == n - 1 ? kEnd : rs[hi]) : __float_as_uint(H.z);
+      // quadrupole builds keep classifier radii in h.zw: the skip pointer is rs[hi], with
+      // hi from the box record (loaded here only then: the group walk's rare fallback)
+      uint32_t nxt = __float_as_uint(H.z);
+      if (quad_records) {
+        const uint32_t hi = __float_as_uint(s.box[u - n].hi.w);
+        nxt = hi == n - 1 ? kEnd : rs[hi];
+      }
       if (!(A.w > 0.0f)) {
         u = nxt;
         continue;
       }
-      const bool inside = (B.x <= me.x) & (me.x <= Cc.x) & (B.y <= me.y) & (me.y <= Cc.y) &
-                          (B.z <= me.z) & (me.z <= Cc.z);
       const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
       const float d2 = dx * dx + dy * dy + dz * dz;
-      if (!inside && d2 >= H.x) {
+      // the box is read only for a node far enough to be accepted (the inside test)
+      bool accept = d2 >= H.x;
+      if (accept) {
+        const float4 B = s.box[u - n].lo, Cc = s.box[u - n].hi;
+        accept = !((B.x <= me.x) & (me.x <= Cc.x) & (B.y <= me.y) & (me.y <= Cc.y) &
+                   (B.z <= me.z) & (me.z <= Cc.z));
+      }
+      if (accept) {
         const float r2 = d2 + eps2;
         const float ir = rsqrtf(r2);
         const float w = A.w * ir * ir * ir;
