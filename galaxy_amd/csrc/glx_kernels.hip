@@ -1953,7 +1953,7 @@ DirectPlan direct_plan(uint32_t n, uint32_t nt, size_t partial_bytes) {
   uint32_t c = (2048u + tblocks - 1) / tblocks;
   const uint32_t by_src = (n + kDsTile - 1) / kDsTile;  // at least one LDS tile per chunk
   if (c > by_src) c = by_src;
-  if (c > 256u) c = 256u;
+  if (c > 2048u) c = 2048u;
   const size_t fit = partial_bytes / (3 * sizeof(double) * (size_t)nt);
   if ((size_t)c > fit) c = (uint32_t)fit;
   if (c < 2u) return p;
