@@ -26,6 +26,8 @@ constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per element tile
 constexpr int kBlock = 256;                            // threads per block of element kernels
 constexpr int kAggTile = 1024;                         // leaves per bottom-up tile (LDS pass)
+constexpr uint32_t kMidTiles = 32;                     // tiles per bottom-up super-tile
+constexpr uint32_t kMidSpan = kMidTiles * kAggTile;    // leaves per super-tile (middle pass)
 constexpr int kGroup = 64;                             // targets per group walk (one wave)
 
 // Walk record of one internal node, split by use into two arrays of 32-B records: the
@@ -88,7 +90,7 @@ enum : uint32_t { kPrecF32 = 0, kPrecKahan = 1, kPrecAbs32 = 2 };
 enum : int {
   kCntInteractions = 0,  // point-mass terms (leaf bodies and monopole nodes), per target
   kCntVisits = 1,        // node tests
-  kCntRoots = 2,         // local subtree roots handed to the top bottom-up pass
+  kCntRoots = 2,         // (unused)
   kCntOverflow = 3,      // group walks that fell back to the per-target walk
   kCntQuad = 4,          // quadrupole-node terms, per target
   kCntMaxStack = 5,      // deepest group-walk stack, entries
@@ -117,7 +119,11 @@ struct SegDesc {
   uint32_t* parent_leaf; // n parent of leaf k (unified)
   uint32_t* rs;          // n-1 right child of the internal node whose split is s
   uint32_t* flag;        // n-1 arrival counters of the top bottom-up pass (kept at 0)
-  uint32_t* roots;       // n local subtree roots handed from the tile pass to the top pass
+  uint32_t* roots;       // n maximal subtrees of each bottom-up tile, per tile at its first
+                         // leaf (rcount[tile] of them), handed to the middle or top pass
+  uint32_t* roots2;      // n the same per super-tile (rcount2 of them), for the top pass
+  uint32_t* rcount;      // ceil(n / kAggTile) roots handed over per tile
+  uint32_t* rcount2;     // ceil(n / kMidSpan) roots handed over per super-tile
   uint32_t* gend;        // n walk groups: last leaf of the group starting at k, else kEnd
   const uint8_t* active; // optional walk mask over the original indices (null: walk all)
   unsigned long long* bbox;      // 6 order-preserving encodings: min xyz, max xyz

@@ -30,8 +30,10 @@ hipError_t launch_flatten(hipStream_t st, const SegDesc* segs, const Tile* et, u
                           const FlatOut& out);
 hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net);
+// Bottom-up pass: at = tiles of kAggTile leaves; mt = super-tiles of kMidSpan leaves of the
+// systems larger than one super-tile (none: nmt = 0).
 hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
-                            const Tile* at, uint32_t nat);
+                            const Tile* at, uint32_t nat, const Tile* mt, uint32_t nmt);
 hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* wt,
                        uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count);
 hipError_t launch_diag(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,

@@ -620,14 +620,18 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 
 // ----------------------------------------------------------------------------------
 // 6. bottom-up aggregate (fold order fixed: left, right; solvers/src/lbvh.rs:623-636),
-//    in two passes that need only workgroup-scope ordering:
+//    in passes that need only workgroup-scope ordering:
 //    (a) per tile of kAggTile sorted leaves, one block climbs from every leaf with
 //        arrival counters in LDS and keeps in-tile node aggregates in LDS; a climb
 //        stops below the first node whose leaf range leaves the tile and hands that
-//        subtree root to (b);
-//    (b) one block per segment climbs from the handed-over roots through the nodes
-//        that straddle tiles, with arrival counters in global memory.
-//    Each completed node is written once as its walk record (and quadrupole).
+//        subtree root on, in the tile's own list;
+//    (b) for systems of more than kMidSpan bodies, one block per super-tile of
+//        kMidTiles tiles climbs from its tiles' roots through the nodes inside the
+//        super-tile, with arrival counters in global memory, and hands on its own roots;
+//    (c) one block per segment climbs from the handed-over roots to the root.
+//    Each completed node is written once as its walk record (and quadrupole). A node's
+//    record depends only on its children's, so the records are the same whichever pass
+//    completes a node and in whatever order its children arrive.
 // ----------------------------------------------------------------------------------
 
 // A pointer from a SegDesc as a global-memory pointer: its loads become global_load
@@ -774,6 +778,7 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
   __shared__ float s_com[3][kAggTile], s_mass[kAggTile], s_mn[3][kAggTile], s_mx[3][kAggTile];
   __shared__ float s_q[NQ][kAggTile];
   __shared__ uint32_t s_flag[kAggTile];
+  __shared__ uint32_t s_nroots;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
@@ -782,6 +787,7 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
     const uint32_t t0 = tl.start, t1 = tl.start + tl.count;
     const float inv_theta = 1.0f / s.theta;
     for (uint32_t q = threadIdx.x; q < kAggTile; q += kAggThreads) s_flag[q] = 0u;
+    if (threadIdx.x == 0) s_nroots = 0u;
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < tl.count; e += kAggThreads) {
       const uint32_t k = t0 + e;
@@ -796,8 +802,9 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
         const uint32_t lo = tj.x, hi = tj.y, cl = left_child(tj.z, n), cr = right_child(tj.z, n);
         const uint32_t pp = tj.w;
         if (lo < t0 || hi >= t1) {
-          const unsigned long long slot = atomicAdd(&s.counters[kCntRoots], 1ull);
-          if (slot < n) s.roots[slot] = u;
+          // u is a maximal subtree of the tile: at most one per tile leaf
+          const uint32_t slot = atomicAdd(&s_nroots, 1u);
+          s.roots[t0 + slot] = u;
           break;
         }
         const uint32_t q = j - t0;
@@ -839,17 +846,132 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
       }
     }
     __syncthreads();
+    if (threadIdx.x == 0) s.rcount[t0 / kAggTile] = s_nroots;
   }
 }
 
 constexpr int kTopThreads = 1024;
 
-// Each lane climbs from one handed-over root at a time and takes the next root from a
-// workgroup counter as soon as its climb ends (first arrival at a node), so the block
-// stays busy and the critical path is the height of the cross-tile part of the tree.
+// The handed-over roots of one pass, as lists: list i holds count[i] roots at
+// roots[i * span ...]. `pre` (in LDS) receives the exclusive prefix of the counts; the
+// block's lanes then take roots one at a time from a workgroup counter over the
+// concatenation, so the work is balanced whatever the list lengths.
+__device__ uint32_t root_prefix(const uint32_t* __restrict__ count, uint32_t nlists,
+                                uint32_t* pre, uint32_t* sh_wave, uint32_t* sh_carry) {
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nlists; c0 += blockDim.x) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < nlists ? count[i] : 0u;
+    const uint32_t ex = block_exclusive_scan(v, sh_wave);
+    if (i < nlists) pre[i] = carry + ex;
+    if (threadIdx.x == blockDim.x - 1) *sh_carry = carry + ex + v;
+    __syncthreads();
+    carry = *sh_carry;
+    __syncthreads();
+  }
+  return carry;
+}
+
+// The list holding concatenated root r: the last i with pre[i] <= r.
+__device__ __forceinline__ uint32_t root_list(const uint32_t* pre, uint32_t nlists, uint32_t r) {
+  uint32_t lo = 0, hi = nlists - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= r) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One climb step of the global passes at node j = p - n: the arrival counter in global
+// memory at workgroup scope (only this block reaches the node), the second arrival folds
+// the children's records and writes the node's. Returns the parent to go on with, or
+// kNoParent when this lane arrived first.
+template <bool QUAD>
+__device__ __forceinline__ uint32_t global_climb(const SegDesc& s, uint32_t j, const uint4& tj,
+                                                 float inv_theta) {
+  const uint32_t n = s.n;
+  const uint32_t lo = tj.x, hi = tj.y, cl = left_child(tj.z, n), cr = right_child(tj.z, n);
+  const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (old == 0u) return kNoParent;
+  __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const uint32_t nx = hi == n - 1 ? kEnd : gbl(s.rs)[hi];
+  const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, cl), global_child<QUAD>(s, cr));
+  store_node<QUAD>(s, j, a, inv_theta, lo, hi, cl, cr, nx);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  return tj.w;
+}
+
+// Middle pass, for systems of more than kMidSpan bodies: one block per super-tile of
+// kMidSpan sorted leaves (kMidTiles tiles) climbs from the roots its tiles handed over,
+// through the nodes whose leaf range stays inside the super-tile (arrival counters in
+// global memory at workgroup scope: no other block reaches them), and hands the roots of
+// its maximal subtrees to the top pass. The top pass is then left with the nodes that
+// straddle super-tiles instead of all those that straddle tiles: on a 2^23-body system
+// one block per super-tile runs instead of one block for the whole cross-tile tree.
+template <bool QUAD>
+__global__ void __launch_bounds__(kTopThreads)
+k_aggregate_mid(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
+                uint32_t ntiles) {
+  __shared__ uint32_t pre[kMidTiles];
+  __shared__ uint32_t sh_wave[kTopThreads / 64], sh_carry;
+  __shared__ uint32_t next_root, s_nout;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const SegDesc& s = segs[tl.seg];
+    const uint32_t n = s.n;
+    const uint32_t S0 = tl.start, S1 = tl.start + tl.count;
+    const uint32_t f = S0 / kAggTile, nl = (tl.count + kAggTile - 1) / kAggTile;
+    if (threadIdx.x == 0) {
+      next_root = 0u;
+      s_nout = 0u;
+    }
+    const uint32_t total = root_prefix(s.rcount + f, nl, pre, sh_wave, &sh_carry);
+    const float inv_theta = 1.0f / s.theta;
+    uint32_t p = kNoParent, cur = 0;
+    for (uint32_t guard = 0; guard < total + tl.count; ++guard) {
+      if (p == kNoParent) {
+        const uint32_t r = atomicAdd(&next_root, 1u);
+        if (r >= total) break;
+        const uint32_t i = root_list(pre, nl, r);
+        cur = s.roots[(f + i) * kAggTile + (r - pre[i])];
+        p = cur < n ? s.parent_leaf[cur] : s.topo[cur - n].w;
+        continue;
+      }
+      const uint32_t j = p - n;
+      if (j >= n - 1) {
+        p = kNoParent;
+        continue;
+      }
+      const uint4 tj = gload_u4(s.topo, j);
+      if (tj.x < S0 || tj.y >= S1) {  // leaves the super-tile: cur goes to the top pass
+        const uint32_t slot = atomicAdd(&s_nout, 1u);
+        s.roots2[S0 + slot] = cur;
+        p = kNoParent;
+        continue;
+      }
+      const uint32_t up = global_climb<QUAD>(s, j, tj, inv_theta);
+      if (up != kNoParent) cur = p;
+      p = up;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s.rcount2[S0 / kMidSpan] = s_nout;
+    __syncthreads();
+  }
+}
+
+// Top pass: one block per system climbs from the roots handed over by the tile pass (or,
+// for systems of more than kMidSpan bodies, by the middle pass) to the root. Each lane
+// climbs from one root at a time and takes the next from a workgroup counter as soon as
+// its climb ends (first arrival at a node), so the block stays busy and the critical path
+// is the height of the part of the tree above the tiles (super-tiles).
+constexpr uint32_t kTopLists = ((uint32_t)1 << 29) / kMidSpan;  // lists of the largest system
+
 template <bool QUAD>
 __global__ void __launch_bounds__(kTopThreads)
 k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
+  __shared__ uint32_t pre[kTopLists];
+  __shared__ uint32_t sh_wave[kTopThreads / 64], sh_carry;
   __shared__ uint32_t next_root;
   for (uint32_t sidx = blockIdx.x; sidx < nseg; sidx += gridDim.x) {
     const SegDesc& s = segs[sidx];
@@ -857,16 +979,20 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
     if (threadIdx.x == 0) next_root = 0u;
     __syncthreads();
     if (n >= 2) {
+      const bool mid = n > kMidSpan;
+      const uint32_t span = mid ? kMidSpan : (uint32_t)kAggTile;
+      const uint32_t nl = (n + span - 1) / span;  // <= kMidTiles without the middle pass
+      const uint32_t* roots = mid ? s.roots2 : s.roots;
+      const uint32_t total = root_prefix(mid ? s.rcount2 : s.rcount, nl, pre, sh_wave, &sh_carry);
       const float inv_theta = 1.0f / s.theta;
-      const unsigned long long cnt64 = s.counters[kCntRoots];
-      const uint32_t cnt = cnt64 < n ? (uint32_t)cnt64 : n;
       uint32_t p = kNoParent;
-      // a lane takes at most cnt roots and climbs at most n-1 nodes in all
-      for (uint32_t guard = 0; guard < cnt + n; ++guard) {
+      // a lane takes at most `total` roots and climbs at most n-1 nodes in all
+      for (uint32_t guard = 0; guard < total + n; ++guard) {
         if (p == kNoParent) {
           const uint32_t r = atomicAdd(&next_root, 1u);
-          if (r >= cnt) break;
-          const uint32_t u = s.roots[r];
+          if (r >= total) break;
+          const uint32_t i = root_list(pre, nl, r);
+          const uint32_t u = roots[i * span + (r - pre[i])];
           p = u < n ? s.parent_leaf[u] : s.topo[u - n].w;
           continue;
         }
@@ -876,25 +1002,9 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
           continue;
         }
         // topology comes from the tree build (an earlier kernel): load it with the arrival
-        const uint4 tj = gload_u4(s.topo, j);
-        const uint32_t lo = tj.x, hi = tj.y, cl = left_child(tj.z, n), cr = right_child(tj.z, n);
-        const uint32_t pp = tj.w;
-        const uint32_t old = __hip_atomic_fetch_add(&s.flag[j], 1u, __ATOMIC_ACQ_REL,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old == 0u) {
-          p = kNoParent;
-          continue;
-        }
-        __hip_atomic_store(&s.flag[j], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t nx = hi == n - 1 ? kEnd : gbl(s.rs)[hi];
-        const AggV a = fold_agg<QUAD>(global_child<QUAD>(s, cl), global_child<QUAD>(s, cr));
-        store_node<QUAD>(s, j, a, inv_theta, lo, hi, cl, cr, nx);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        p = pp;
+        p = global_climb<QUAD>(s, j, gload_u4(s.topo, j), inv_theta);
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s.counters[kCntRoots] = 0ull;
     __syncthreads();
   }
 }
@@ -1885,13 +1995,16 @@ hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, co
 }
 
 hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
-                            const Tile* at, uint32_t nat) {
+                            const Tile* at, uint32_t nat, const Tile* mt, uint32_t nmt) {
   const dim3 tg(grid_for(nat)), tb(kAggThreads), sg(grid_for(nseg)), sb(kTopThreads);
+  const dim3 mg(grid_for(nmt));
   if (md.quad) {
     hipLaunchKernelGGL(k_aggregate_tile<true>, tg, tb, 0, st, segs, at, nat);
+    if (nmt) hipLaunchKernelGGL(k_aggregate_mid<true>, mg, sb, 0, st, segs, mt, nmt);
     hipLaunchKernelGGL(k_aggregate_top<true>, sg, sb, 0, st, segs, nseg);
   } else {
     hipLaunchKernelGGL(k_aggregate_tile<false>, tg, tb, 0, st, segs, at, nat);
+    if (nmt) hipLaunchKernelGGL(k_aggregate_mid<false>, mg, sb, 0, st, segs, mt, nmt);
     hipLaunchKernelGGL(k_aggregate_top<false>, sg, sb, 0, st, segs, nseg);
   }
   return hipGetLastError();

@@ -72,7 +72,7 @@ struct Carve {
 
 struct Layout {
   size_t x, v, m, acc, acc_out, k0, k1, v0, v1, leaf, node, box, quad, topo, pleaf, rs, flag,
-      roots, gend, bbox, bounds, counters, total;
+      roots, roots2, rcount, rcount2, gend, bbox, bounds, counters, total;
 };
 
 Layout layout_for(uint32_t n, bool with_acc_out) {
@@ -97,6 +97,9 @@ Layout layout_for(uint32_t n, bool with_acc_out) {
   L.rs = c.take<uint32_t>(N);
   L.flag = c.take<uint32_t>(N);
   L.roots = c.take<uint32_t>(N);
+  L.roots2 = c.take<uint32_t>(N);
+  L.rcount = c.take<uint32_t>((N + kAggTile - 1) / kAggTile);
+  L.rcount2 = c.take<uint32_t>((N + kMidSpan - 1) / kMidSpan);
   L.gend = c.take<uint32_t>(N);
   L.bbox = c.take<unsigned long long>(6);
   L.bounds = c.take<double>(8);
@@ -154,6 +157,9 @@ void Segment::reserve(uint32_t n, bool with_acc_out) {
   d.rs = reinterpret_cast<uint32_t*>(b + L.rs);
   d.flag = reinterpret_cast<uint32_t*>(b + L.flag);
   d.roots = reinterpret_cast<uint32_t*>(b + L.roots);
+  d.roots2 = reinterpret_cast<uint32_t*>(b + L.roots2);
+  d.rcount = reinterpret_cast<uint32_t*>(b + L.rcount);
+  d.rcount2 = reinterpret_cast<uint32_t*>(b + L.rcount2);
   d.gend = reinterpret_cast<uint32_t*>(b + L.gend);
   d.bbox = reinterpret_cast<unsigned long long*>(b + L.bbox);
   d.bounds = reinterpret_cast<double*>(b + L.bounds);
@@ -184,6 +190,7 @@ Batch::~Batch() {
   dev_free(segs_);
   dev_free(etiles_);
   dev_free(atiles_);
+  dev_free(mtiles_);
   dev_free(wtiles_);
   dev_free(gtiles_);
   dev_free(tile_hist_);
@@ -212,18 +219,20 @@ static void push_tiles(std::vector<Tile>& v, uint32_t seg, uint32_t n, uint32_t 
 void Batch::set_segments(const std::vector<SegDesc>& descs) {
   host_ = descs;
   nseg_ = (uint32_t)descs.size();
-  std::vector<Tile> et, at, wt, gt;
+  std::vector<Tile> et, at, mt, wt, gt;
   for (uint32_t s = 0; s < nseg_; ++s) {
     SegDesc& d = host_[s];
     d.sort_tile0 = (uint32_t)et.size();
     push_tiles(et, s, d.n, kSortTile);
     d.sort_tiles = (uint32_t)et.size() - d.sort_tile0;
     push_tiles(at, s, d.n, kAggTile);
+    if (d.n > kMidSpan) push_tiles(mt, s, d.n, kMidSpan);
     push_tiles(wt, s, d.n, kBlock);
     push_tiles(gt, s, d.n, kGroup);
   }
   n_et_ = (uint32_t)et.size();
   n_at_ = (uint32_t)at.size();
+  n_mt_ = (uint32_t)mt.size();
   n_wt_ = (uint32_t)wt.size();
   n_gt_ = (uint32_t)gt.size();
   size_t cs = cap_segs_;
@@ -252,6 +261,7 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   }
   cap_et_ = ce;
   grow(atiles_, cap_at_, n_at_ ? n_at_ : 1);
+  grow(mtiles_, cap_mt_, n_mt_ ? n_mt_ : 1);
   grow(wtiles_, cap_wt_, n_wt_ ? n_wt_ : 1);
   grow(gtiles_, cap_gt_, n_gt_ ? n_gt_ : 1);
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -261,6 +271,7 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   up(segs_, host_.data(), nseg_ * sizeof(SegDesc));
   up(etiles_, et.data(), n_et_ * sizeof(Tile));
   up(atiles_, at.data(), n_at_ * sizeof(Tile));
+  up(mtiles_, mt.data(), n_mt_ * sizeof(Tile));
   up(wtiles_, wt.data(), n_wt_ * sizeof(Tile));
   up(gtiles_, gt.data(), n_gt_ * sizeof(Tile));
   check(hipStreamSynchronize(stream_), "set_segments");
@@ -306,7 +317,7 @@ void Batch::refresh_leaves() {
 }
 void Batch::karras() { check(launch_karras(stream_, mode_, segs_, etiles_, n_et_), "karras"); }
 void Batch::aggregate() {
-  check(launch_aggregate(stream_, mode_, segs_, nseg_, atiles_, n_at_), "aggregate");
+  check(launch_aggregate(stream_, mode_, segs_, nseg_, atiles_, n_at_, mtiles_, n_mt_), "aggregate");
 }
 void Batch::flatten(const FlatOut& out) {
   check(launch_flatten(stream_, segs_, etiles_, n_et_, out), "flatten");

@@ -310,6 +310,15 @@ def test_aggregate_deterministic_on_a_deep_cross_tile_tree():
                 assert np.array_equal(a[k], b[k]), f"{k} not deterministic (quad={quad})"
 
 
+def test_aggregate_super_tile_edges():
+    """Systems just above one super-tile of 32768 leaves (the middle bottom-up pass runs
+    for them) and ragged ones whose last super-tile holds a single leaf or part of a tile:
+    AABBs bit-exact against the oracle, the rest to the aggregate tolerances."""
+    for n in (32769, 32768 * 2 + 1000, 32768 * 3 + 1):
+        pos = I.cloud(n, n, 3.0)
+        assert_aggregate(*_sorted_inputs(pos, I.masses(n ^ 0x55, n)), 3.0, f"cloud{n}")
+
+
 def test_aggregate_large_and_deterministic():
     rng = np.random.default_rng(12)
     pos = rng.standard_normal((1 << 18, 3)) * 4.0
