@@ -28,6 +28,12 @@ constexpr int kBlock = 256;                            // threads per block of e
 constexpr int kAggTile = 1024;                         // leaves per bottom-up tile (LDS pass)
 constexpr uint32_t kMidTiles = 32;                     // tiles per bottom-up super-tile
 constexpr uint32_t kMidSpan = kMidTiles * kAggTile;    // leaves per super-tile (middle pass)
+#ifndef GLX_MID_MIN
+#define GLX_MID_MIN (8u * kMidSpan)
+#endif
+// Systems of more than kMidMin bodies run the middle bottom-up pass (below it the top
+// pass alone is faster than two dependent climbing kernels).
+constexpr uint32_t kMidMin = GLX_MID_MIN;
 constexpr int kGroup = 64;                             // targets per group walk (one wave)
 
 // Walk record of one internal node, split by use into two arrays of 32-B records: the

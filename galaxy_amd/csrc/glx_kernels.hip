@@ -669,8 +669,12 @@ __device__ __forceinline__ AggV leaf_agg(float4 p) {
   return r;
 }
 
+// The fold and the record below are written out without floating-point contraction, so
+// a node's record is the same bits whichever kernel computes it (the compiler would
+// otherwise fuse multiplies and adds differently in each inlining context).
 template <bool QUAD>
 __device__ __forceinline__ AggV fold_agg(const AggV& l, const AggV& r) {
+#pragma clang fp contract(off)
   AggV o;
   o.mass = l.mass + r.mass;
   if (o.mass > 0.0f) {
@@ -712,6 +716,7 @@ template <bool QUAD>
 __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const AggV& r,
                                            float inv_theta, uint32_t lo, uint32_t hi,
                                            uint32_t cl, uint32_t cr, uint32_t nx) {
+#pragma clang fp contract(off)
   const float cx = (r.mn.x + r.mx.x) * 0.5f, cy = (r.mn.y + r.mx.y) * 0.5f,
               cz = (r.mn.z + r.mx.z) * 0.5f;
   const float hx = (r.mx.x - r.mn.x) * 0.5f, hy = (r.mx.y - r.mn.y) * 0.5f,
@@ -766,87 +771,169 @@ __device__ __forceinline__ AggV global_child(const SegDesc& s, uint32_t c) {
   return r;
 }
 
-// One thread per leaf of the tile: every climb starts at once, so the critical path
-// is the depth of the tile's subtrees.
+// Tile pass, one block of one thread per leaf per tile, in two phases:
+//  A. every leaf climbs with arrival words in LDS, as far as the nodes inside the tile
+//     go; an arrival only adds its subtree height, and the second arrival at a node
+//     learns the node's height (1 + the taller child's) and climbs on. The nodes that
+//     stop a climb by leaving the tile get the climb's subtree root handed on.
+//  B. the in-tile nodes, bucketed by height in LDS, are folded one height at a time
+//     (every child of a height-h node is lower), so the lanes that fold are packed into
+//     as few waves as the bucket needs: the fold, most of a climb step's instructions,
+//     runs about once per node instead of once per wave per tree level.
+// Topology is staged in LDS first by coalesced loads (an in-tile node j has t0 <= j < t1:
+// an internal node's leaf range has its own index at one end), packed tile-relative in
+// 8 bytes; each leaf is staged into its parent's LDS record, which the parent's fold
+// reads before overwriting it with its own aggregate.
 constexpr int kAggThreads = kAggTile;
+// Heights of in-tile nodes stay below this: the common prefix of a node's range (key bits,
+// then index bits for equal keys: at most 64 + 32) grows strictly from parent to child.
+constexpr uint32_t kMaxHeight = 128;
+// packed in-tile topology (s_info): x = lo | hi << 10 | inside | parent << 21 | parent in
+// tile; y = split | root | leaf flags (kLeftLeaf, kRightLeaf), all tile-relative
+constexpr uint32_t kInInside = 1u << 20, kInParent = 1u << 31, kInRoot = 1u << 20;
+
+// Block barrier that orders LDS only: the global stores before it (node records, read by
+// later kernels only) are not waited for, as __syncthreads' release fence would.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 template <bool QUAD>
-__global__ void __launch_bounds__(kAggThreads)
+__device__ __forceinline__ AggV lds_child(const float4 (*agg)[kAggTile], uint32_t c) {
+  AggV r;
+  const float4 p0 = agg[0][c], p1 = agg[1][c], p2 = agg[2][c];
+  r.com = make_float3(p0.x, p0.y, p0.z);
+  r.mass = p0.w;
+  r.mn = make_float3(p1.x, p1.y, p1.z);
+  r.mx = make_float3(p2.x, p2.y, p2.z);
+  if (QUAD) {
+    const float4 p3 = agg[QUAD ? 3 : 0][c];
+    r.q[0] = p1.w; r.q[1] = p2.w; r.q[2] = p3.x; r.q[3] = p3.y; r.q[4] = p3.z; r.q[5] = p3.w;
+  } else {
+    for (int k = 0; k < 6; ++k) r.q[k] = 0.f;
+  }
+  return r;
+}
+
+// Two blocks per CU (the LDS above fits twice into 160 KB): 8 waves per SIMD, so at most
+// 64 VGPRs.
+template <bool QUAD>
+__global__ void __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
                  uint32_t ntiles) {
-  constexpr int NQ = QUAD ? 6 : 1;
-  __shared__ float s_com[3][kAggTile], s_mass[kAggTile], s_mn[3][kAggTile], s_mx[3][kAggTile];
-  __shared__ float s_q[NQ][kAggTile];
-  __shared__ uint32_t s_flag[kAggTile];
-  __shared__ uint32_t s_nroots;
+  constexpr int NP = QUAD ? 4 : 3;  // com+mass, min(+q0), max(+q1), q2..q5
+  __shared__ float4 s_agg[NP][kAggTile];
+  __shared__ uint2 s_info[kAggTile];
+  __shared__ uint32_t s_hw[kAggTile];  // A: arrivals << 16 | height; B: nodes by height
+  __shared__ uint32_t s_hoff[kMaxHeight + 1], s_hcur[kMaxHeight];
+  __shared__ uint32_t s_nroots, s_maxh;
+  const uint32_t e = threadIdx.x;  // this thread's leaf (and staged node) in the tile
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const uint32_t n = s.n;
     if (n < 2) continue;
-    const uint32_t t0 = tl.start, t1 = tl.start + tl.count;
+    const uint32_t t0 = tl.start, cnt = tl.count, t1 = t0 + cnt;
     const float inv_theta = 1.0f / s.theta;
-    for (uint32_t q = threadIdx.x; q < kAggTile; q += kAggThreads) s_flag[q] = 0u;
-    if (threadIdx.x == 0) s_nroots = 0u;
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < tl.count; e += kAggThreads) {
-      const uint32_t k = t0 + e;
-      uint32_t u = k;
-      uint32_t p = gbl(s.parent_leaf)[k];
-      // each ancestor is visited once; the bound only guards corrupted input
-      for (uint32_t guard = 0; p != kNoParent && guard < n; ++guard) {
-        const uint32_t j = p - n;
-        if (j >= n - 1) break;
-        // all topology of node j in one 16-B load
-        const uint4 tj = gload_u4(s.topo, j);
-        const uint32_t lo = tj.x, hi = tj.y, cl = left_child(tj.z, n), cr = right_child(tj.z, n);
-        const uint32_t pp = tj.w;
-        if (lo < t0 || hi >= t1) {
-          // u is a maximal subtree of the tile: at most one per tile leaf
-          const uint32_t slot = atomicAdd(&s_nroots, 1u);
-          s.roots[t0 + slot] = u;
+    if (e < kMaxHeight) s_hcur[e] = 0u;
+    if (e == 0) s_nroots = 0u;
+    float4 lf = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t p = kNoParent;
+    if (e < cnt) {
+      lf = gload4(s.leaf, t0 + e);
+      p = gbl(s.parent_leaf)[t0 + e];
+      uint2 inf = make_uint2(0u, 0u);
+      if (t0 + e < n - 1) {
+        const uint4 tj = gload_u4(s.topo, t0 + e);
+        if (tj.x >= t0 && tj.y < t1) {
+          const uint32_t pr = tj.w - n - t0;  // wraps far past cnt for kNoParent
+          inf.x = (tj.x - t0) | (tj.y - t0) << 10 | kInInside;
+          if (tj.w != kNoParent && pr < cnt) inf.x |= pr << 21 | kInParent;
+          inf.y = ((tj.z & kSplitMask) - t0) | (tj.z & (kLeftLeaf | kRightLeaf)) |
+                  (tj.w == kNoParent ? kInRoot : 0u);
+        }
+      }
+      s_info[e] = inf;
+      s_hw[e] = 0u;
+    }
+    lds_barrier();
+    // A: climbs (each in-tile node is reached twice; the bound only guards corrupted input)
+    if (e < cnt) {
+      uint32_t u = t0 + e, hu = 0;
+      for (uint32_t guard = 0; guard < cnt; ++guard) {
+        const uint32_t q = p - n - t0;
+        const uint32_t ix = q < cnt ? s_info[q].x : 0u;
+        if (!(ix & kInInside)) {  // u is a maximal subtree of the tile
+          s.roots[t0 + atomicAdd(&s_nroots, 1u)] = u;
           break;
         }
-        const uint32_t q = j - t0;
-        // Siblings hand over their aggregates through LDS only (the node records written
-        // to global memory are not read in this pass), so the release/acquire pair is
-        // limited to LDS and does not wait for the global stores.
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        const uint32_t old =
-            __hip_atomic_fetch_add(&s_flag[q], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        if (old == 0u) break;
-        const uint32_t nx = hi == n - 1 ? kEnd : gbl(s.rs)[hi];
-        AggV ch[2];
-        const uint32_t cs[2] = {cl, cr};
-        for (int c = 0; c < 2; ++c) {
-          if (cs[c] < n) {
-            const auto* lp = gbl(reinterpret_cast<const float*>(s.leaf)) + 4 * (size_t)cs[c];
-            ch[c] = leaf_agg(make_float4(lp[0], lp[1], lp[2], lp[3]));
-          } else {
-            const uint32_t cq = cs[c] - n - t0;
-            AggV& r = ch[c];
-            r.com = make_float3(s_com[0][cq], s_com[1][cq], s_com[2][cq]);
-            r.mass = s_mass[cq];
-            r.mn = make_float3(s_mn[0][cq], s_mn[1][cq], s_mn[2][cq]);
-            r.mx = make_float3(s_mx[0][cq], s_mx[1][cq], s_mx[2][cq]);
-            for (int w = 0; w < 6; ++w) r.q[w] = QUAD ? s_q[QUAD ? w : 0][cq] : 0.f;
-          }
+        if (hu == 0u)  // u is a leaf: stage it for its parent's fold (left child = split)
+          s_agg[(s_info[q].y & 1023u) == e ? 0 : 1][q] = lf;
+        const uint32_t old = atomicAdd(&s_hw[q], 0x10000u + hu + 1u);
+        if (old < 0x10000u) break;  // first arrival
+        const uint32_t h = max(old & 0xFFFFu, hu + 1u);
+        const uint32_t hb = h < kMaxHeight ? h : kMaxHeight - 1;
+        s_hw[q] = hb;
+        atomicAdd(&s_hcur[hb], 1u);
+        hu = h;
+        u = n + t0 + q;
+        if (ix & kInParent) {
+          p = n + t0 + ((ix >> 21) & 1023u);
+          continue;
         }
-        const AggV r = fold_agg<QUAD>(ch[0], ch[1]);
-        s_com[0][q] = r.com.x; s_com[1][q] = r.com.y; s_com[2][q] = r.com.z;
-        s_mass[q] = r.mass;
-        s_mn[0][q] = r.mn.x; s_mn[1][q] = r.mn.y; s_mn[2][q] = r.mn.z;
-        s_mx[0][q] = r.mx.x; s_mx[1][q] = r.mx.y; s_mx[2][q] = r.mx.z;
-        if (QUAD)
-          for (int w = 0; w < 6; ++w) s_q[QUAD ? w : 0][q] = r.q[w];
-        store_node<QUAD>(s, j, r, inv_theta, lo, hi, cl, cr, nx);
-        u = p;
-        p = pp;
+        if (!(s_info[q].y & kInRoot)) s.roots[t0 + atomicAdd(&s_nroots, 1u)] = u;
+        break;
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s.rcount[t0 / kAggTile] = s_nroots;
+    lds_barrier();
+    // the in-tile nodes in buckets by height: offsets, then each node's slot
+    const uint32_t hn = (e < cnt && (s_info[e].x & kInInside)) ? s_hw[e] : 0u;
+    if (e < 64) {
+      const uint32_t c0 = s_hcur[2 * e], c1 = s_hcur[2 * e + 1], v = c0 + c1;
+      uint32_t inc = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t w = __shfl_up(inc, o);
+        if ((int)e >= o) inc += w;
+      }
+      s_hoff[2 * e] = inc - v;
+      s_hoff[2 * e + 1] = inc - v + c0;
+      s_hcur[2 * e] = inc - v;
+      s_hcur[2 * e + 1] = inc - v + c0;
+      if (e == 63) s_hoff[kMaxHeight] = inc;
+      // the tallest height present: the last non-empty bucket
+      const unsigned long long any = __ballot(v != 0u);
+      if (e == 0) s_maxh = any ? 2u * (63u - (uint32_t)__clzll(any)) + 1u : 0u;
+    }
+    lds_barrier();
+    if (hn) s_hw[atomicAdd(&s_hcur[hn], 1u)] = e;
+    lds_barrier();
+    // B: folds, one height at a time
+    const uint32_t H = s_maxh;
+    for (uint32_t h = 1; h <= H; ++h) {
+      const uint32_t idx = s_hoff[h] + e;
+      if (idx < s_hoff[h + 1]) {
+        const uint32_t q = s_hw[idx];
+        const uint2 inf = s_info[q];
+        const uint32_t g = inf.y & 1023u;
+        const bool ll = inf.y & kLeftLeaf, rl = inf.y & kRightLeaf;
+        const AggV a = ll ? leaf_agg(s_agg[0][q]) : lds_child<QUAD>(s_agg, g);
+        const AggV b = rl ? leaf_agg(s_agg[1][q]) : lds_child<QUAD>(s_agg, g + 1);
+        const AggV r = fold_agg<QUAD>(a, b);
+        s_agg[0][q] = make_float4(r.com.x, r.com.y, r.com.z, r.mass);
+        s_agg[1][q] = make_float4(r.mn.x, r.mn.y, r.mn.z, r.q[0]);
+        s_agg[2][q] = make_float4(r.mx.x, r.mx.y, r.mx.z, r.q[1]);
+        if (QUAD) s_agg[QUAD ? 3 : 0][q] = make_float4(r.q[2], r.q[3], r.q[4], r.q[5]);
+        const uint32_t lo = t0 + (inf.x & 1023u), hi = t0 + ((inf.x >> 10) & 1023u);
+        const uint32_t cl = ll ? t0 + g : n + t0 + g, cr = rl ? t0 + g + 1 : n + t0 + g + 1;
+        const uint32_t nx = QUAD ? 0u : (hi == n - 1 ? kEnd : gbl(s.rs)[hi]);
+        store_node<QUAD>(s, t0 + q, r, inv_theta, lo, hi, cl, cr, nx);
+      }
+      lds_barrier();
+    }
+    if (e == 0) s.rcount[t0 / kAggTile] = s_nroots;
+    lds_barrier();
   }
 }
 
@@ -979,9 +1066,9 @@ k_aggregate_top(const SegDesc* __restrict__ segs, uint32_t nseg) {
     if (threadIdx.x == 0) next_root = 0u;
     __syncthreads();
     if (n >= 2) {
-      const bool mid = n > kMidSpan;
+      const bool mid = n > kMidMin;
       const uint32_t span = mid ? kMidSpan : (uint32_t)kAggTile;
-      const uint32_t nl = (n + span - 1) / span;  // <= kMidTiles without the middle pass
+      const uint32_t nl = (n + span - 1) / span;  // <= kMidMin / kAggTile without the middle pass
       const uint32_t* roots = mid ? s.roots2 : s.roots;
       const uint32_t total = root_prefix(mid ? s.rcount2 : s.rcount, nl, pre, sh_wave, &sh_carry);
       const float inv_theta = 1.0f / s.theta;

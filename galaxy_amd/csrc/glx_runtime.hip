@@ -226,7 +226,7 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
     push_tiles(et, s, d.n, kSortTile);
     d.sort_tiles = (uint32_t)et.size() - d.sort_tile0;
     push_tiles(at, s, d.n, kAggTile);
-    if (d.n > kMidSpan) push_tiles(mt, s, d.n, kMidSpan);
+    if (d.n > kMidMin) push_tiles(mt, s, d.n, kMidSpan);
     push_tiles(wt, s, d.n, kBlock);
     push_tiles(gt, s, d.n, kGroup);
   }

@@ -228,6 +228,11 @@ def test_aggregate_reference_cases():
     n = 2048
     assert_aggregate(np.arange(n, dtype=np.uint32), I.cloud(0xC0FFEE, n, 4.0), I.masses(0xBEE, n), 4.0,
                      "chain")
+    # one key per bit (0, 1, 2, 4, ..., 2^29), each 33 times: a caterpillar of 30 levels
+    # over balanced tie subtrees, about 36 levels inside one tile
+    cat = np.repeat(np.array([0] + [1 << k for k in range(30)], np.uint32), 33)
+    m = cat.size
+    assert_aggregate(cat, I.cloud(0xCA7, m, 2.0), I.masses(0xCA8, m), 2.0, "caterpillar")
     g = glx.aggregate(np.array([5], np.uint32), np.array([[2.0, -1.0, 0.5]]), np.array([1.25]))
     assert g["mass"][0] == np.float32(1.25)
     assert np.array_equal(g["aabb_min"][0] + g["origin"].astype(np.float32),
@@ -311,12 +316,18 @@ def test_aggregate_deterministic_on_a_deep_cross_tile_tree():
 
 
 def test_aggregate_super_tile_edges():
-    """Systems just above one super-tile of 32768 leaves (the middle bottom-up pass runs
-    for them) and ragged ones whose last super-tile holds a single leaf or part of a tile:
-    AABBs bit-exact against the oracle, the rest to the aggregate tolerances."""
-    for n in (32769, 32768 * 2 + 1000, 32768 * 3 + 1):
+    """Systems above 2^18 bodies, for which the middle bottom-up pass runs over super-tiles
+    of 32768 leaves, with a last super-tile of a single leaf and one ending inside a tile:
+    AABBs bit-exact against the oracle, the rest to the aggregate tolerances, and the
+    quadrupole build bit-identical over repeated builds."""
+    for n in (262145, 262144 + 3 * 32768 + 517):
         pos = I.cloud(n, n, 3.0)
-        assert_aggregate(*_sorted_inputs(pos, I.masses(n ^ 0x55, n)), 3.0, f"cloud{n}")
+        args = _sorted_inputs(pos, I.masses(n ^ 0x55, n))
+        assert_aggregate(*args, 3.0, f"cloud{n}")
+        a = glx.aggregate(*args, quad=True)
+        b = glx.aggregate(*args, quad=True)
+        for k in ("aabb_min", "aabb_max", "com", "mass", "quad"):
+            assert np.array_equal(a[k], b[k]), f"{k} not deterministic (n={n})"
 
 
 def test_aggregate_large_and_deterministic():
