@@ -73,16 +73,17 @@ WALK_VALU_PER_CU_CYCLE = 1.02
 
 
 def phase_bytes(morton_bits, radix_bits, quad):
-    """Algorithmic HBM bytes per particle of each HBM-bound phase: kick-drift reads and
-    writes x (f64) and v (f32); encode reads x and writes key + payload; the LSD sort reads
-    the keys once for the first histogram, then reads and writes key + payload in each of
-    ceil(bits / radix_bits) passes; the gather reads the order and a 16-B leaf source and
-    writes a 16-B leaf; Karras reads the codes and writes children and parents; the
-    bottom-up pass writes and reads back each node record (40 B, 64 B with quadrupoles)."""
+    """Algorithmic HBM bytes per particle of each HBM-bound phase: kick-drift reads v and a
+    (f32) and x (f64) and writes v and x; encode reads x and writes key + payload; each of
+    the ceil(bits / radix_bits) LSD sort passes reads the keys for its histogram, then reads
+    and writes key + payload; the gather reads the order, x and m (f64) and writes a 16-B
+    leaf; Karras reads the codes and writes the 16-B topology record, a leaf parent and a
+    skip entry; the bottom-up pass reads the leaf, its parent and the topology and writes
+    the 32-B classifier and 32-B box records (and the 32-B quadrupole)."""
     kb = 4 if morton_bits == 30 else 8
     passes = -(-(30 if morton_bits == 30 else 63) // (radix_bits or 8))
-    return {"kick_drift_bbox": 72, "encode": 28 + kb, "sort": kb + 2 * (kb + 4) * passes,
-            "gather": 36, "karras": 16 + kb, "aggregate": 264 if quad else 168}
+    return {"kick_drift_bbox": 84, "encode": 28 + kb, "sort": (3 * kb + 8) * passes,
+            "gather": 52, "karras": 24 + kb, "aggregate": 132 if quad else 100}
 
 
 HBM_PEAK_GBS = 8000.0
