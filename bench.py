@@ -28,7 +28,7 @@ reference CPU path).
 Roofline: the walk (k_walk_group) dominates; it is FP32-VALU bound. `achieved` counts 20
 flops per interaction of the reference algorithm on the same positions (counted by the
 reference build); the executed figure counts 20 flops per point-mass term, 56 per
-quadrupole term, 150 per far node and 60 per body of a local expansion (terms counted by
+quadrupole term, 230 per far node and 140 per body of a local expansion (terms counted by
 the kernel); both over the walk's average device time from HIP events on the pool's
 stream.
 Prints ONE JSON line.
@@ -51,24 +51,24 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector with packed FP32; 78.6 without
 FP32_NONPACKED_TFLOPS = 78.6
 FLOPS_POINT = 20
 FLOPS_QUAD = 56
-# far field: one node folded into a group's local expansion (95 VALU operations, FMAs
-# counted twice) and the evaluation of the expansion at one body
-FLOPS_M2L = 150
-FLOPS_L2P = 60
+# far field: one node folded into a group's third-order local expansion (about 140 VALU
+# operations, FMAs counted twice) and the evaluation of the expansion at one body
+FLOPS_M2L = 230
+FLOPS_L2P = 140
 # HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
-# `bench.py --profile --leaf-size 1 --far-theta 0.55 --mono-ratio 0.8 --group-size 64
+# `bench.py --profile --leaf-size 1 --far-theta 0.6 --mono-ratio 0.9 --group-size 64
 # --group-lanes 4 --radix-bits 8 --morton-bits 30` (--pmc FETCH_SIZE, then WRITE_SIZE),
 # scaled by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`):
-# FETCH_SIZE reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 31.28 GB fetched
-# + 0.56 GB written; the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 31.278e9 + 0.559e9
+# FETCH_SIZE reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 30.76 GB fetched
+# + 0.57 GB written; the tested node records come from beyond L2.
+WALK_TRAFFIC_BYTES = 30.760e9 + 0.569e9
 # (leaf size, far_theta, mono_ratio, group size, lanes per body, key bits) the walk counters were
 # measured on
-WALK_TRAFFIC_CONFIG = (1, 0.55, 0.8, 64, 4, 30)
+WALK_TRAFFIC_CONFIG = (1, 0.6, 0.9, 64, 4, 30)
 # SQ_INSTS_VALU / SQ_BUSY_CU_CYCLES of the walk, same config: VALU wave-instructions per
 # CU-cycle; one per CU-cycle (four SIMDs each taking four cycles per wave64 instruction) is
 # the issue limit, so the walk is VALU-issue bound at whatever clock the GPU runs
-WALK_VALU_PER_CU_CYCLE = 1.02
+WALK_VALU_PER_CU_CYCLE = 1.00
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 
 
@@ -698,12 +698,12 @@ def main():
             "frac_vs_non_packed_peak": b["walk"]["tflops_ref"] / FP32_NONPACKED_TFLOPS,
             "i_ref_per_particle": b["walk"]["i_ref_per_particle"],
             # the terms the kernel executes (20 per point term, 56 per quadrupole term,
-            # 150 per far node, 60 per body of a local expansion)
+            # 230 per far node, 140 per body of a local expansion)
             "achieved_executed": b["walk"]["tflops"],
             "frac_executed": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
             "traffic": WALK_TRAFFIC_BYTES if knobs == WALK_TRAFFIC_CONFIG else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
-                              " leaf size 1, far-field ratio 0.55, monopole band 0.8, groups"
+                              " leaf size 1, far-field ratio 0.6, monopole band 0.9, groups"
                               " of 64, 4 lanes, 30-bit keys",
             "flops_per_launch": b["walk"]["flops_ref"],
             "executed_flops_per_launch": b["walk"]["flops"],

@@ -1396,18 +1396,23 @@ __device__ __forceinline__ void quad_term(Acc3& a, const float4& me, const float
 }
 
 // Far field of a group: a node (com at d from the group centre, mass m, traceless Q) is
-// folded into a second-order local expansion of the softened acceleration about the
-// centre, a(c + y) = A0 + A1 y + A2 y y / 2 (R^2 = |d|^2 + eps^2):
+// folded into a local expansion of the softened acceleration about the centre,
+// a(c + y) = A0 + A1 y + A2 y y / 2 + A3 y y y / 6 (R^2 = |d|^2 + eps^2):
 //   monopole   A0 = m d / R^3,  A1_ij = m (3 d_i d_j / R^5 - delta_ij / R^3),
 //              A2_ijk = m (15 d_i d_j d_k / R^7
-//                          - 3 (delta_ij d_k + delta_ik d_j + delta_jk d_i) / R^5)
+//                          - 3 (delta_ij d_k + delta_ik d_j + delta_jk d_i) / R^5),
+//              A3_ijkl = m (105 d_i d_j d_k d_l / R^9 - 15 (delta_ij d_k d_l + 5 more
+//                           pairings) / R^7 + 3 (delta_ij delta_kl + 2 more) / R^5)
 //   quadrupole A0 = -q / R^5 + 2.5 s d / R^7 (q = Q d, s = d.q),
 //              A1_ij = Q_ij / R^5 - 5 (q_i d_j + q_j d_i) / R^7 - 2.5 s delta_ij / R^7
 //                      + 17.5 s d_i d_j / R^9
-// (the quadrupole's second-order term is dropped: it is smaller than the neglected
-// third-order monopole term for well-separated nodes). Coefficients: L[0..2] = A0,
-// L[3..8] = A1 {xx,yy,zz,xy,xz,yz}, L[9..18] = A2 {xxx,xxy,xxz,xyy,xyz,xzz,yyy,yyz,yzz,zzz}.
-constexpr int kLocal = 19;
+// (the successive derivatives of the field in y: each order adds one factor |y| / R to
+// the truncation error; the quadrupole's second-order term is dropped: it is smaller than
+// the neglected fourth-order monopole term for well-separated nodes). Coefficients:
+// L[0..2] = A0, L[3..8] = A1 {xx,yy,zz,xy,xz,yz},
+// L[9..18] = A2 {xxx,xxy,xxz,xyy,xyz,xzz,yyy,yyz,yzz,zzz},
+// L[19..33] = A3 {xxxx,xxxy,xxxz,xxyy,xxyz,xxzz,xyyy,xyyz,xyzz,xzzz,yyyy,yyyz,yyzz,yzzz,zzzz}.
+constexpr int kLocal = 34;
 
 __device__ __forceinline__ void far_m2l(float* L, float x, float y, float z, float m,
                                         const float4& Q0, const float4& Q1, float eps2) {
@@ -1437,6 +1442,24 @@ __device__ __forceinline__ void far_m2l(float* L, float x, float y, float z, flo
   L[16] += z * fmaf(w, yy, -t);
   L[17] += y * fmaf(w, zz, -t);
   L[18] += z * fmaf(w, zz, -3.0f * t);
+  // third order: 105 m X / R^9 - w (sum over equal-index pairs of the other two d's)
+  // + t (number of pairings into equal pairs), X the component's product of four d's
+  const float a9 = 7.0f * w * ir2;  // 105 m / R^9
+  L[19] += fmaf(a9 * xx, xx, fmaf(-6.0f * w, xx, 3.0f * t));
+  L[20] += xy * fmaf(a9, xx, -3.0f * w);
+  L[21] += xz * fmaf(a9, xx, -3.0f * w);
+  L[22] += fmaf(a9 * xx, yy, fmaf(-w, xx + yy, t));
+  L[23] += yz * fmaf(a9, xx, -w);
+  L[24] += fmaf(a9 * xx, zz, fmaf(-w, xx + zz, t));
+  L[25] += xy * fmaf(a9, yy, -3.0f * w);
+  L[26] += xz * fmaf(a9, yy, -w);
+  L[27] += xy * fmaf(a9, zz, -w);
+  L[28] += xz * fmaf(a9, zz, -3.0f * w);
+  L[29] += fmaf(a9 * yy, yy, fmaf(-6.0f * w, yy, 3.0f * t));
+  L[30] += yz * fmaf(a9, yy, -3.0f * w);
+  L[31] += fmaf(a9 * yy, zz, fmaf(-w, yy + zz, t));
+  L[32] += yz * fmaf(a9, zz, -3.0f * w);
+  L[33] += fmaf(a9 * zz, zz, fmaf(-6.0f * w, zz, 3.0f * t));
   // quadrupole {xx,yy,zz,xy} = Q0, {xz,yz} = Q1.xy
   const float qx = fmaf(Q0.x, x, fmaf(Q0.w, y, Q1.x * z));
   const float qy = fmaf(Q0.w, x, fmaf(Q0.y, y, Q1.y * z));
@@ -1454,7 +1477,8 @@ __device__ __forceinline__ void far_m2l(float* L, float x, float y, float z, flo
   L[8] += fmaf(Q1.y, ir5, e * yz) - g5 * fmaf(qy, z, qz * y);
 }
 
-// a(c + y) from the local coefficients.
+// a(c + y) from the local coefficients: a_i = A0_i + A1_ij y_j + A2_ijk y_j y_k / 2
+// + A3_ijkl y_j y_k y_l / 6, the symmetric sums written over distinct monomials.
 __device__ __forceinline__ void far_l2p(const float* L, float x, float y, float z, float& ax,
                                         float& ay, float& az) {
   const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
@@ -1467,6 +1491,16 @@ __device__ __forceinline__ void far_l2p(const float* L, float x, float y, float 
   az = L[2] + fmaf(L[7], x, fmaf(L[8], y, L[5] * z)) +
        0.5f * fmaf(L[11], xx, fmaf(L[16], yy, fmaf(L[18], zz,
               2.0f * fmaf(L[13], xy, fmaf(L[14], xz, L[17] * yz)))));
+  // third order over the cubic monomials scaled by 1 / (p! q! r!)
+  const float c0 = xx * x * (1.0f / 6.0f), c1 = 0.5f * xx * y, c2 = 0.5f * xx * z,
+              c3 = 0.5f * x * yy, c4 = xy * z, c5 = 0.5f * x * zz, c6 = yy * y * (1.0f / 6.0f),
+              c7 = 0.5f * yy * z, c8 = 0.5f * y * zz, c9 = zz * z * (1.0f / 6.0f);
+  ax += fmaf(L[19], c0, fmaf(L[20], c1, fmaf(L[21], c2, fmaf(L[22], c3, fmaf(L[23], c4,
+        fmaf(L[24], c5, fmaf(L[25], c6, fmaf(L[26], c7, fmaf(L[27], c8, L[28] * c9)))))))));
+  ay += fmaf(L[20], c0, fmaf(L[22], c1, fmaf(L[23], c2, fmaf(L[25], c3, fmaf(L[26], c4,
+        fmaf(L[27], c5, fmaf(L[29], c6, fmaf(L[30], c7, fmaf(L[31], c8, L[32] * c9)))))))));
+  az += fmaf(L[21], c0, fmaf(L[23], c1, fmaf(L[24], c2, fmaf(L[26], c3, fmaf(L[27], c4,
+        fmaf(L[28], c5, fmaf(L[30], c6, fmaf(L[31], c7, fmaf(L[32], c8, L[33] * c9)))))))));
 }
 
 // The queued far nodes (node indices) into this lane's share of the local expansion:
@@ -1503,8 +1537,12 @@ __device__ __forceinline__ uint32_t bucket_terms(Acc3& a, const float4& me, cons
   return total;
 }
 
+// The far-field build holds a 34-coefficient local expansion per lane: without a bound
+// the compiler takes about 147 VGPRs (3 waves per SIMD: walk +21%); at 4 waves per SIMD it
+// fits 128 VGPRs without spilling. The counting build is left unbounded (it only counts).
 template <bool KAHAN, bool COUNT, bool QUAD, bool FAR>
 __global__ void __launch_bounds__(kGroup)
+__attribute__((amdgpu_waves_per_eu(COUNT ? 1 : 4)))
 k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
              int mode) {
   constexpr int QN = QUAD ? kListCap : 1;

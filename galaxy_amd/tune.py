@@ -6,8 +6,9 @@ Accuracy knobs of the group walk (each candidate is error-checked):
   body by body instead of opened further — larger buckets trade quadrupole terms and node
   tests for cheaper point-mass terms; they never raise the error.
 * far_theta: an accepted node well separated from the whole group (node radius + group
-  radius <= far_theta x distance to the group centre) enters the group's local expansion
-  instead of one term per body — fewer operations, a larger error as far_theta grows.
+  radius <= far_theta x distance to the group centre) enters the group's third-order
+  local expansion instead of one term per body — fewer operations, a larger error as
+  far_theta grows.
 * mono_ratio: an accepted node that also passes the opening test at theta x mono_ratio is
   summed as a monopole alone (14 operations per body instead of 33) — the quadrupole is
   kept for the nearer accepted nodes, where it matters.
@@ -57,7 +58,7 @@ from .pool import Pool
 
 CANDIDATES = (1, 2, 4, 8, 16)
 LEAF_SIZES = (1, 2)
-FAR_THETAS = (0.0, 0.5, 0.55, 0.6)
+FAR_THETAS = (0.0, 0.55, 0.6, 0.65, 0.7)
 MONO_RATIOS = (0.0, 0.7, 0.8, 0.9)
 GROUP_SHAPES = ((64, 4), (64, 1), (48, 4), (32, 4))
 RADIX_WIDTHS = (8, 9, 10)

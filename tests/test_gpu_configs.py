@@ -30,8 +30,8 @@ from tests import inputs as I
 pytestmark = pytest.mark.gpu
 REF = dict(morton_bits=30, walk=glx.WALK_REFERENCE, multipole=0)
 FAST = dict(morton_bits=63, walk=glx.WALK_GROUP, multipole=2)
-ENSEMBLE = dict(morton_bits=30, walk=glx.WALK_GROUP, multipole=2, leaf_size=1, far_theta=0.55,
-                mono_ratio=0.8, group_size=64, group_lanes=4, radix_bits=8)
+ENSEMBLE = dict(morton_bits=30, walk=glx.WALK_GROUP, multipole=2, leaf_size=1, far_theta=0.6,
+                mono_ratio=0.9, group_size=64, group_lanes=4, radix_bits=8)
 
 
 def say(*a):
