@@ -24,6 +24,8 @@ constexpr int kRadixBinsMax = 1 << kRadixBitsMax;
 constexpr int kSortThreads = 512;
 constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per element tile
+constexpr uint32_t kScanChunk = 32;     // sort tiles per chunk of the multi-block digit scan
+constexpr uint32_t kScanChunkMin = 256; // segments of more sort tiles than this scan in chunks
 constexpr int kBlock = 256;                            // threads per block of element kernels
 constexpr int kAggTile = 1024;                         // leaves per bottom-up tile (LDS pass)
 constexpr uint32_t kMidTiles = 32;                     // tiles per bottom-up super-tile
@@ -137,6 +139,7 @@ struct SegDesc {
   unsigned long long* counters;  // kCounters: see kCnt*
   uint32_t n;
   uint32_t sort_tile0, sort_tiles;  // element tiles (kSortTile) of this segment
+  uint32_t scan_chunk0, scan_chunks;  // its chunks of kScanChunk tiles (0: scanned whole)
   uint32_t precision;
   uint32_t leaf_size;  // group walk: nodes of at most this many bodies are summed body by body
   uint32_t group_size;  // group walk: bodies per walk group (subtrees of at most this many)

@@ -21,8 +21,17 @@ hipError_t launch_encode(hipStream_t st, const Mode& md, const SegDesc* segs, co
 hipError_t launch_bounds_only(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg);
 hipError_t launch_iota(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                        uint32_t net);
+// Per-batch scratch of the sort: per-tile digit histograms and offsets (2^B per tile), and
+// the chunk table and per-chunk rows of the segments scanned in chunks (nchunks may be 0).
+struct SortScratch {
+  uint32_t* tile_hist;
+  uint32_t* tile_off;
+  const Tile* chunks;
+  uint32_t nchunks;
+  uint32_t* chunk_rows;
+};
 hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
-                       const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off);
+                       const Tile* et, uint32_t net, const SortScratch& sc);
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net);
 hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
                                  uint32_t net);

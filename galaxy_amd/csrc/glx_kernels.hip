@@ -344,16 +344,20 @@ k_radix_upsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles
   }
 }
 
-// One block of 1024 threads per segment: thread (digit d, chunk g) scans one of the
-// 1024 / 2^B chunks of the segment's tiles; tile offsets = digit base (exclusive scan
-// over digits of the digit totals) + exclusive prefix over earlier tiles.
+// One block of 1024 threads per segment: thread (digit d, group g) scans one of the
+// 1024 / 2^B groups of the segment's rows; row offsets = digit base (exclusive scan over
+// digits of the digit totals) + exclusive prefix over earlier rows. The rows are the
+// segment's tiles, or, for a segment of more than kScanChunkMin tiles, its chunks of
+// kScanChunk tiles: their sums come from k_radix_chunk_sum and the offsets found here
+// replace them in place (k_radix_chunk_offsets then spreads them over the tiles), so
+// that no thread walks more than a few dozen rows in a row.
 constexpr int kScanThreads = 1024;
 
 template <int B>
 __global__ void __launch_bounds__(kScanThreads)
 k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg,
-             const uint32_t* __restrict__ tile_hist,
-             uint32_t* __restrict__ tile_off) {
+             const uint32_t* __restrict__ tile_hist, uint32_t* __restrict__ tile_off,
+             uint32_t* chunk_rows) {
   constexpr int BINS = 1 << B, CHUNKS = kScanThreads / BINS;
   __shared__ uint32_t part[CHUNKS][BINS];
   __shared__ uint32_t base[BINS];
@@ -361,11 +365,15 @@ k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg,
   const uint32_t d = threadIdx.x & (BINS - 1), g = threadIdx.x / BINS;
   for (uint32_t sidx = blockIdx.x; sidx < nseg; sidx += gridDim.x) {
     const SegDesc& s = segs[sidx];
-    const uint32_t nt = s.sort_tiles, per = (nt + CHUNKS - 1) / CHUNKS;
-    const uint32_t c0 = s.sort_tile0 + min(nt, g * per), c1 = s.sort_tile0 + min(nt, (g + 1) * per);
+    const bool chunked = s.scan_chunks > 0;
+    const uint32_t* in = chunked ? chunk_rows : tile_hist;
+    uint32_t* out = chunked ? chunk_rows : tile_off;  // chunk offsets replace their sums
+    const uint32_t r0 = chunked ? s.scan_chunk0 : s.sort_tile0;
+    const uint32_t nt = chunked ? s.scan_chunks : s.sort_tiles, per = (nt + CHUNKS - 1) / CHUNKS;
+    const uint32_t c0 = r0 + min(nt, g * per), c1 = r0 + min(nt, (g + 1) * per);
     uint32_t sum = 0;
 #pragma unroll 8
-    for (uint32_t t = c0; t < c1; ++t) sum += tile_hist[(size_t)t * BINS + d];
+    for (uint32_t t = c0; t < c1; ++t) sum += in[(size_t)t * BINS + d];
     part[g][d] = sum;
     __syncthreads();
     if (threadIdx.x < BINS) {
@@ -385,12 +393,52 @@ k_radix_scan(const SegDesc* __restrict__ segs, uint32_t nseg,
     for (uint32_t w = 0; w < (d >> 6); ++w) run += wsum[w];
     for (uint32_t q = 0; q < g; ++q) run += part[q][d];
 #pragma unroll 8
-    for (uint32_t t = c0; t < c1; ++t) {
-      const uint32_t h = tile_hist[(size_t)t * BINS + d];
-      tile_off[(size_t)t * BINS + d] = run;
+    for (uint32_t t = c0; t < c1; ++t) {  // each row read before this thread overwrites it
+      const uint32_t h = in[(size_t)t * BINS + d];
+      out[(size_t)t * BINS + d] = run;
       run += h;
     }
     __syncthreads();
+  }
+}
+
+// The chunks of the segments scanned in chunks: Tile{segment, first sort tile (of the
+// batch), tiles, -}, chunk c's row at c * 2^B.
+constexpr int kChunkThreads = 256;
+
+template <int B>
+__global__ void __launch_bounds__(kChunkThreads)
+k_radix_chunk_sum(const Tile* __restrict__ chunks, uint32_t nchunks,
+                  const uint32_t* __restrict__ tile_hist, uint32_t* __restrict__ chunk_rows) {
+  constexpr int BINS = 1 << B;
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const Tile ch = chunks[c];
+    for (uint32_t d = threadIdx.x; d < (uint32_t)BINS; d += kChunkThreads) {
+      uint32_t sum = 0;
+#pragma unroll 8
+      for (uint32_t t = ch.start; t < ch.start + ch.count; ++t) sum += tile_hist[(size_t)t * BINS + d];
+      chunk_rows[(size_t)c * BINS + d] = sum;
+    }
+  }
+}
+
+template <int B>
+__global__ void __launch_bounds__(kChunkThreads)
+k_radix_chunk_offsets(const Tile* __restrict__ chunks, uint32_t nchunks,
+                      const uint32_t* __restrict__ tile_hist,
+                      const uint32_t* __restrict__ chunk_rows, uint32_t* __restrict__ tile_off) {
+  constexpr int BINS = 1 << B;
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const Tile ch = chunks[c];
+    for (uint32_t d = threadIdx.x; d < (uint32_t)BINS; d += kChunkThreads) {
+      uint32_t run = chunk_rows[(size_t)c * BINS + d];
+#pragma unroll 8
+      for (uint32_t t = ch.start; t < ch.start + ch.count; ++t) {
+        const uint32_t h = tile_hist[(size_t)t * BINS + d];
+        tile_off[(size_t)t * BINS + d] = run;
+        run += h;
+      }
+    }
   }
 }
 
@@ -2055,15 +2103,23 @@ hipError_t launch_iota(hipStream_t st, const Mode& md, const SegDesc* segs, cons
 // The keys start in buffer sort_start() and end in buffer 0.
 template <class K, int B>
 static void sort_passes(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
-                        uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+                        uint32_t net, const SortScratch& sc) {
   constexpr bool key63 = sizeof(K) == 8;
+  uint32_t* tile_hist = sc.tile_hist;
+  uint32_t* tile_off = sc.tile_off;
   int src = sort_start(key63, B);
   for (int pass = 0; pass < sort_passes_for(key63, B); ++pass) {
     const int shift = pass * B;
     hipLaunchKernelGGL((k_radix_upsweep<K, B>), dim3(grid_for(net)), dim3(kSortThreads), 0, st,
                        segs, et, net, tile_hist, shift, src);
+    if (sc.nchunks)
+      hipLaunchKernelGGL(k_radix_chunk_sum<B>, dim3(grid_for(sc.nchunks)), dim3(kChunkThreads), 0,
+                         st, sc.chunks, sc.nchunks, tile_hist, sc.chunk_rows);
     hipLaunchKernelGGL(k_radix_scan<B>, dim3(grid_for(nseg)), dim3(kScanThreads), 0, st, segs,
-                       nseg, tile_hist, tile_off);
+                       nseg, tile_hist, tile_off, sc.chunk_rows);
+    if (sc.nchunks)
+      hipLaunchKernelGGL(k_radix_chunk_offsets<B>, dim3(grid_for(sc.nchunks)), dim3(kChunkThreads),
+                         0, st, sc.chunks, sc.nchunks, tile_hist, sc.chunk_rows, tile_off);
     hipLaunchKernelGGL((k_radix_downsweep<K, B>), dim3(grid_for(net)), dim3(kSortThreads), 0, st,
                        segs, et, net, tile_off, shift, src);
     src ^= 1;
@@ -2072,22 +2128,22 @@ static void sort_passes(hipStream_t st, const SegDesc* segs, uint32_t nseg, cons
 
 template <class K>
 static void sort_width(hipStream_t st, int bits, const SegDesc* segs, uint32_t nseg,
-                       const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+                       const Tile* et, uint32_t net, const SortScratch& sc) {
   if (bits == 10)
-    sort_passes<K, 10>(st, segs, nseg, et, net, tile_hist, tile_off);
+    sort_passes<K, 10>(st, segs, nseg, et, net, sc);
   else if (bits == 9)
-    sort_passes<K, 9>(st, segs, nseg, et, net, tile_hist, tile_off);
+    sort_passes<K, 9>(st, segs, nseg, et, net, sc);
   else
-    sort_passes<K, 8>(st, segs, nseg, et, net, tile_hist, tile_off);
+    sort_passes<K, 8>(st, segs, nseg, et, net, sc);
 }
 
 hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
-                       const Tile* et, uint32_t net, uint32_t* tile_hist, uint32_t* tile_off) {
+                       const Tile* et, uint32_t net, const SortScratch& sc) {
   if (md.radix_bits < 8 || md.radix_bits > kRadixBitsMax) return hipErrorInvalidValue;
   if (md.key63)
-    sort_width<uint64_t>(st, md.radix_bits, segs, nseg, et, net, tile_hist, tile_off);
+    sort_width<uint64_t>(st, md.radix_bits, segs, nseg, et, net, sc);
   else
-    sort_width<uint32_t>(st, md.radix_bits, segs, nseg, et, net, tile_hist, tile_off);
+    sort_width<uint32_t>(st, md.radix_bits, segs, nseg, et, net, sc);
   return hipGetLastError();
 }
 

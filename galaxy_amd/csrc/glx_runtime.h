@@ -106,14 +106,17 @@ class Batch {
   Tile* etiles_ = nullptr;  // kSortTile elements per tile
   Tile* atiles_ = nullptr;  // kAggTile leaves per tile (bottom-up pass)
   Tile* mtiles_ = nullptr;  // kMidSpan leaves per super-tile (systems larger than one)
+  Tile* ctiles_ = nullptr;  // kScanChunk sort tiles per chunk (segments scanned in chunks)
+  uint32_t* chunk_rows_ = nullptr;  // per chunk: digit sums, then digit offsets
   Tile* wtiles_ = nullptr;  // kBlock targets per tile (per-target walk)
   Tile* gtiles_ = nullptr;  // kGroup targets per tile (group walk)
-  uint32_t n_et_ = 0, n_at_ = 0, n_mt_ = 0, n_wt_ = 0, n_gt_ = 0;
+  uint32_t n_et_ = 0, n_at_ = 0, n_mt_ = 0, n_ct_ = 0, n_wt_ = 0, n_gt_ = 0;
   uint32_t* tile_hist_ = nullptr;
   uint32_t* tile_off_ = nullptr;
   double* diag_partial_ = nullptr;
   double* diag_out_ = nullptr;
-  size_t cap_segs_ = 0, cap_et_ = 0, cap_at_ = 0, cap_mt_ = 0, cap_wt_ = 0, cap_gt_ = 0;
+  size_t cap_segs_ = 0, cap_et_ = 0, cap_at_ = 0, cap_mt_ = 0, cap_ct_ = 0, cap_wt_ = 0,
+         cap_gt_ = 0;
 };
 
 }  // namespace glx

@@ -191,6 +191,8 @@ Batch::~Batch() {
   dev_free(etiles_);
   dev_free(atiles_);
   dev_free(mtiles_);
+  dev_free(ctiles_);
+  dev_free(chunk_rows_);
   dev_free(wtiles_);
   dev_free(gtiles_);
   dev_free(tile_hist_);
@@ -219,12 +221,17 @@ static void push_tiles(std::vector<Tile>& v, uint32_t seg, uint32_t n, uint32_t 
 void Batch::set_segments(const std::vector<SegDesc>& descs) {
   host_ = descs;
   nseg_ = (uint32_t)descs.size();
-  std::vector<Tile> et, at, mt, wt, gt;
+  std::vector<Tile> et, at, mt, ct, wt, gt;
   for (uint32_t s = 0; s < nseg_; ++s) {
     SegDesc& d = host_[s];
     d.sort_tile0 = (uint32_t)et.size();
     push_tiles(et, s, d.n, kSortTile);
     d.sort_tiles = (uint32_t)et.size() - d.sort_tile0;
+    d.scan_chunk0 = (uint32_t)ct.size();
+    if (d.sort_tiles > kScanChunkMin)  // chunks of kScanChunk of the segment's sort tiles
+      for (uint32_t c = 0; c < d.sort_tiles; c += kScanChunk)
+        ct.push_back(Tile{s, d.sort_tile0 + c, d.sort_tiles - c < kScanChunk ? d.sort_tiles - c : kScanChunk, 0});
+    d.scan_chunks = (uint32_t)ct.size() - d.scan_chunk0;
     push_tiles(at, s, d.n, kAggTile);
     if (d.n > kMidMin) push_tiles(mt, s, d.n, kMidSpan);
     push_tiles(wt, s, d.n, kBlock);
@@ -233,6 +240,7 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   n_et_ = (uint32_t)et.size();
   n_at_ = (uint32_t)at.size();
   n_mt_ = (uint32_t)mt.size();
+  n_ct_ = (uint32_t)ct.size();
   n_wt_ = (uint32_t)wt.size();
   n_gt_ = (uint32_t)gt.size();
   size_t cs = cap_segs_;
@@ -262,6 +270,15 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   cap_et_ = ce;
   grow(atiles_, cap_at_, n_at_ ? n_at_ : 1);
   grow(mtiles_, cap_mt_, n_mt_ ? n_mt_ : 1);
+  size_t cc = cap_ct_;
+  grow(ctiles_, cc, n_ct_ ? n_ct_ : 1);
+  if (cc != cap_ct_ || !chunk_rows_) {
+    dev_free(chunk_rows_);
+    chunk_rows_ = nullptr;
+    dev_malloc(reinterpret_cast<void**>(&chunk_rows_), cc * kRadixBinsMax * sizeof(uint32_t),
+               "hipMalloc(scan chunks)");
+  }
+  cap_ct_ = cc;
   grow(wtiles_, cap_wt_, n_wt_ ? n_wt_ : 1);
   grow(gtiles_, cap_gt_, n_gt_ ? n_gt_ : 1);
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -272,6 +289,7 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   up(etiles_, et.data(), n_et_ * sizeof(Tile));
   up(atiles_, at.data(), n_at_ * sizeof(Tile));
   up(mtiles_, mt.data(), n_mt_ * sizeof(Tile));
+  up(ctiles_, ct.data(), n_ct_ * sizeof(Tile));
   up(wtiles_, wt.data(), n_wt_ * sizeof(Tile));
   up(gtiles_, gt.data(), n_gt_ * sizeof(Tile));
   check(hipStreamSynchronize(stream_), "set_segments");
@@ -309,7 +327,8 @@ void Batch::encode(uint32_t* lanes_out) {
 void Batch::bounds_only() { check(launch_bounds_only(stream_, mode_, segs_, nseg_), "bounds"); }
 void Batch::iota() { check(launch_iota(stream_, mode_, segs_, etiles_, n_et_), "iota"); }
 void Batch::sort() {
-  check(launch_sort(stream_, mode_, segs_, nseg_, etiles_, n_et_, tile_hist_, tile_off_), "sort");
+  const SortScratch sc{tile_hist_, tile_off_, ctiles_, n_ct_, chunk_rows_};
+  check(launch_sort(stream_, mode_, segs_, nseg_, etiles_, n_et_, sc), "sort");
 }
 void Batch::gather() { check(launch_gather(stream_, segs_, etiles_, n_et_), "gather"); }
 void Batch::refresh_leaves() {
