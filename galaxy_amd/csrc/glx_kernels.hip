@@ -582,23 +582,36 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 // read global memory.
 constexpr int kKarrasPad = 512;
 
+// A pointer from a SegDesc as a global-memory pointer: its loads become global_load
+// (counted by vmcnt only) instead of flat loads, which also hold up every LDS wait.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 template <class K>
 struct KeyWindow {
   const K* codes;
   const K* win;
   int w0, wlen;
+  // The window as an LDS read (clamped index) and the global key only outside it, so
+  // neither becomes a flat load through a selected pointer.
   __device__ __forceinline__ K at(int g) const {
     const int q = g - w0;
-    return (q >= 0 && q < wlen) ? win[q] : codes[g];
+    const bool in = q >= 0 && q < wlen;
+    K v = win[in ? q : 0];
+    if (!in) v = gbl(codes)[g];
+    return v;
   }
 };
 
+// delta(a, b) of the node's own key ka (position a) and the key at b; -1 outside [0, n)
 template <class K>
-__device__ __forceinline__ int kdelta(const KeyWindow<K>& kw, int n, int a, int b) {
+__device__ __forceinline__ int kdelta(const KeyWindow<K>& kw, int n, int a, K ka, int b) {
   if (b < 0 || b >= n) return -1;
-  const K ca = kw.at(a), cb = kw.at(b);
-  if (ca == cb) return KeyTraits<K>::tie_base + __clz((int)((uint32_t)a ^ (uint32_t)b));
-  return KeyTraits<K>::clz(ca ^ cb);
+  const K cb = kw.at(b);
+  if (ka == cb) return KeyTraits<K>::tie_base + __clz((int)((uint32_t)a ^ (uint32_t)b));
+  return KeyTraits<K>::clz(ka ^ cb);
 }
 
 // 1024 threads per 2048-key tile: two nodes per thread, so the binary searches of a tile
@@ -623,19 +636,20 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
     for (uint32_t e = threadIdx.x; e < tl.count; e += kKarrasThreads) {
       const int i = (int)(tl.start + e);
       if (i >= n - 1) continue;
-      const int dir = kdelta(kw, n, i, i + 1) > kdelta(kw, n, i, i - 1) ? 1 : -1;
-      const int dmin = kdelta(kw, n, i, i - dir);
+      const K ki = kw.at(i);
+      const int dir = kdelta(kw, n, i, ki, i + 1) > kdelta(kw, n, i, ki, i - 1) ? 1 : -1;
+      const int dmin = kdelta(kw, n, i, ki, i - dir);
       int lmax = 2;
-      while (kdelta(kw, n, i, i + lmax * dir) > dmin) lmax *= 2;
+      while (kdelta(kw, n, i, ki, i + lmax * dir) > dmin) lmax *= 2;
       int l = 0;
       for (int tt = lmax / 2; tt > 0; tt /= 2)
-        if (kdelta(kw, n, i, i + (l + tt) * dir) > dmin) l += tt;
+        if (kdelta(kw, n, i, ki, i + (l + tt) * dir) > dmin) l += tt;
       const int j = i + l * dir;
-      const int dnode = kdelta(kw, n, i, j);
+      const int dnode = kdelta(kw, n, i, ki, j);
       int sp = 0, tt = l;
       for (;;) {
         tt = (tt + 1) / 2;
-        if (kdelta(kw, n, i, i + (sp + tt) * dir) > dnode) sp += tt;
+        if (kdelta(kw, n, i, ki, i + (sp + tt) * dir) > dnode) sp += tt;
         if (tt <= 1) break;
       }
       const int gamma = i + sp * dir + (dir < 0 ? dir : 0);
@@ -681,13 +695,6 @@ k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 //    record depends only on its children's, so the records are the same whichever pass
 //    completes a node and in whatever order its children arrive.
 // ----------------------------------------------------------------------------------
-
-// A pointer from a SegDesc as a global-memory pointer: its loads become global_load
-// (counted by vmcnt only) instead of flat loads, which also hold up every LDS wait.
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
-  return (const __attribute__((address_space(1))) T*)p;
-}
 
 // 16-byte record i of a float4 array as one global load (a flat load would also hold up
 // the LDS waits that follow it).
