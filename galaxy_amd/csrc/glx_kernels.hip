@@ -174,6 +174,18 @@ __device__ __forceinline__ uint32_t quant_lane(double v, double top) {
   return f >= top ? (uint32_t)top : (uint32_t)f;
 }
 
+// A pointer from a SegDesc as a global-memory pointer: its loads and stores become
+// global_load / global_store (counted by vmcnt only) instead of flat ones, which also
+// hold up every LDS wait.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gblw(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 // ----------------------------------------------------------------------------------
 // 1. bounding box
 // ----------------------------------------------------------------------------------
@@ -210,16 +222,19 @@ k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
     const SegDesc& s = segs[tl.seg];
     const float h = s.half_dt;
     const double dt = s.dt;
+    auto* sv = gblw(s.v);
+    auto* sx = gblw(s.x);
+    const auto* sa = gbl(s.acc);
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     if (s.precision == kPrecAbs32) {  // the all-f32 path drifts in f32 (tile-uniform)
       const float dt32 = (float)dt;
       for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
         const size_t i = tl.start + e;
         for (int a = 0; a < 3; ++a) {
-          const float v = s.v[3 * i + a] + s.acc[3 * i + a] * h;
-          s.v[3 * i + a] = v;
-          const double x = (double)((float)s.x[3 * i + a] + v * dt32);
-          s.x[3 * i + a] = x;
+          const float v = sv[3 * i + a] + sa[3 * i + a] * h;
+          sv[3 * i + a] = v;
+          const double x = (double)((float)sx[3 * i + a] + v * dt32);
+          sx[3 * i + a] = x;
           lo[a] = fmin(lo[a], x);
           hi[a] = fmax(hi[a], x);
         }
@@ -228,10 +243,10 @@ k_kick_drift_bbox(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
       for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
         const size_t i = tl.start + e;
         for (int a = 0; a < 3; ++a) {
-          const float v = s.v[3 * i + a] + s.acc[3 * i + a] * h;
-          s.v[3 * i + a] = v;
-          const double x = s.x[3 * i + a] + (double)v * dt;
-          s.x[3 * i + a] = x;
+          const float v = sv[3 * i + a] + sa[3 * i + a] * h;
+          sv[3 * i + a] = v;
+          const double x = sx[3 * i + a] + (double)v * dt;
+          sx[3 * i + a] = x;
           lo[a] = fmin(lo[a], x);
           hi[a] = fmax(hi[a], x);
         }
@@ -269,13 +284,15 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       s.bounds[6] = abs32 ? 0.0 : b.origin[1];
       s.bounds[7] = abs32 ? 0.0 : b.origin[2];
     }
-    K* keys = static_cast<K*>(s.keys[dst]);  // the sort's first source buffer
-    uint32_t* vals = s.vals[dst];
+    auto* keys = gblw(static_cast<K*>(s.keys[dst]));  // the sort's first source buffer
+    auto* vals = gblw(s.vals[dst]);
+    auto* gend = gblw(s.gend);
+    const auto* sx = gbl(s.x);
     for (uint32_t e = threadIdx.x; e < tl.count; e += kBlock) {
       const uint32_t i = tl.start + e;
       uint32_t l[3];
       for (int a = 0; a < 3; ++a)
-        l[a] = quant_lane((s.x[3 * (size_t)i + a] - b.bmin[a]) * b.scale, top);
+        l[a] = quant_lane((sx[3 * (size_t)i + a] - b.bmin[a]) * b.scale, top);
       if (lanes_out) {
         lanes_out[3 * (size_t)i] = l[0];
         lanes_out[3 * (size_t)i + 1] = l[1];
@@ -285,7 +302,7 @@ k_encode(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
       vals[i] = i;
       // a fresh build: group starts are marked by the tree build, which runs after the
       // sort (concurrently with the gather, which therefore leaves the marks alone)
-      s.gend[i] = s.n == 1 ? 0u : kEnd;
+      gend[i] = s.n == 1 ? 0u : kEnd;
       if (s.n == 1) s.parent_leaf[0] = kNoParent;
     }
   }
@@ -332,7 +349,7 @@ k_radix_upsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    const K* keys = static_cast<const K*>(s.keys[src]) + tl.start;
+    const auto* keys = gbl(static_cast<const K*>(s.keys[src]) + tl.start);
     for (int d = threadIdx.x; d < BINS; d += kSortThreads) hist[d] = 0;
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < tl.count; e += kSortThreads)
@@ -463,10 +480,10 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
-    const K* keys_in = static_cast<const K*>(s.keys[src]) + tl.start;
-    const uint32_t* vals_in = s.vals[src] + tl.start;
-    K* keys_out = static_cast<K*>(s.keys[src ^ 1]);
-    uint32_t* vals_out = s.vals[src ^ 1];
+    const auto* keys_in = gbl(static_cast<const K*>(s.keys[src]) + tl.start);
+    const auto* vals_in = gbl(s.vals[src] + tl.start);
+    auto* keys_out = gblw(static_cast<K*>(s.keys[src ^ 1]));
+    auto* vals_out = gblw(s.vals[src ^ 1]);
     for (int k = 0; k < W; ++k)
       for (int d = threadIdx.x; d < BINS; d += kSortThreads) wcnt[k][d] = 0;
     __syncthreads();
@@ -560,13 +577,13 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
     const double o0 = s.bounds[5], o1 = s.bounds[6], o2 = s.bounds[7];
     for (uint32_t e = threadIdx.x; e < tl.count; e += kGatherThreads) {
       const uint32_t k = tl.start + e;
-      const size_t i = min(s.vals[0][k], s.n - 1);
-      float4 lf;
-      lf.x = (float)(s.x[3 * i] - o0);
-      lf.y = (float)(s.x[3 * i + 1] - o1);
-      lf.z = (float)(s.x[3 * i + 2] - o2);
-      lf.w = (float)s.m[i];
-      s.leaf[k] = lf;
+      const size_t i = min(gbl(s.vals[0])[k], s.n - 1);
+      const auto* x = gbl(s.x);
+      auto* lp = gblw(reinterpret_cast<float*>(s.leaf)) + 4 * (size_t)k;
+      lp[0] = (float)(x[3 * i] - o0);
+      lp[1] = (float)(x[3 * i + 1] - o1);
+      lp[2] = (float)(x[3 * i + 2] - o2);
+      lp[3] = (float)gbl(s.m)[i];
     }
   }
 }
@@ -581,13 +598,6 @@ k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint3
 // of most nodes stay inside this window and only the long ranges of nodes near the root
 // read global memory.
 constexpr int kKarrasPad = 512;
-
-// A pointer from a SegDesc as a global-memory pointer: its loads become global_load
-// (counted by vmcnt only) instead of flat loads, which also hold up every LDS wait.
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
-  return (const __attribute__((address_space(1))) T*)p;
-}
 
 template <class K>
 struct KeyWindow {
@@ -1577,16 +1587,21 @@ __device__ __forceinline__ float wave_sumf(float v) {
 }
 
 // Bodies of the buckets in `rl[0, nr)` (first leaf, count) on the lane's target. The
-// range is wave-uniform, so the body records come through scalar loads.
+// range is wave-uniform, so the body records come through scalar loads (the leaf array
+// is read-only during the walk: a constant-address-space pointer gives s_load).
 template <bool KAHAN>
 __device__ __forceinline__ uint32_t bucket_terms(Acc3& a, const float4& me, const SegDesc& s,
                                                  const uint2* rl, uint32_t nr, float eps2) {
+  const auto* leaf = (const __attribute__((address_space(4))) float*)s.leaf;
   uint32_t total = 0;
   for (uint32_t r = 0; r < nr; ++r) {
     const uint2 rg = rl[r];
     const uint32_t lo = __builtin_amdgcn_readfirstlane(rg.x);
     const uint32_t c = __builtin_amdgcn_readfirstlane(rg.y);
-    for (uint32_t e = 0; e < c; ++e) point_term<KAHAN>(a, me, s.leaf[lo + e], eps2);
+    for (uint32_t e = 0; e < c; ++e) {
+      const auto* q = leaf + 4 * (size_t)(lo + e);
+      point_term<KAHAN>(a, me, make_float4(q[0], q[1], q[2], q[3]), eps2);
+    }
     total += c;
   }
   return total;
