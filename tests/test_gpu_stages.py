@@ -107,9 +107,10 @@ def test_sort_bit_exact_reference_cases():
 
 def test_sort_bit_exact_million_and_ragged():
     """Sizes around the sort tile (2048 keys) and above 256 tiles, where the digit scan runs
-    over chunks of 32 tiles (2^20: 16 whole chunks; 301 tiles: a last chunk of 13)."""
+    over chunks of 32 tiles (2^20: 16 whole chunks; 301 tiles: a last chunk of 13; 257
+    tiles: a last chunk of one tile; 256 tiles: scanned whole)."""
     rng = np.random.default_rng(9)
-    for n in (1 << 20, 2048 * 300 + 5, 2048 * 3 + 17, 2047, 2049, 255):
+    for n in (1 << 20, 2048 * 300 + 5, 2048 * 256 + 1, 2048 * 256, 2048 * 3 + 17, 2047, 2049, 255):
         codes = rng.integers(0, 1 << 30, n, dtype=np.uint32)
         assert_sort_exact(codes, f"uniform{n}")
     codes = (rng.integers(0, 50, 1 << 18, dtype=np.uint32) << 20).astype(np.uint32)
