@@ -1581,6 +1581,12 @@ __device__ __forceinline__ void far_drain(float* L, const SegDesc& s, const uint
   }
 }
 
+// Lanes below this one with their bit set in the wave mask m (v_mbcnt_lo/hi: two VALU
+// instructions, against an AND and two bit counts on a per-lane mask).
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ float wave_sumf(float v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -1623,7 +1629,6 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
   __shared__ uint2 rl[kRangeCap];
   __shared__ uint32_t fl[FAR ? kFarCap : 1];
   const uint32_t lane = threadIdx.x;
-  const unsigned long long lt = (1ull << lane) - 1ull;
   for (uint32_t t = xcd_block(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
@@ -1761,23 +1766,23 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           overflow = true;
           break;
         }
-        if (point) pl[np + __popcll(m1 & lt)] = A;
+        if (point) pl[np + lanes_below(m1)] = A;
         if (QUAD && quadn) {  // only accepted nodes fetch their moments
-          const uint32_t q = nq + __popcll(m2 & lt);
+          const uint32_t q = nq + lanes_below(m2);
           const float4 q1 = gload4(s.quad, 2 * (size_t)ni + 1);
           qa[q] = A;
           qb[q] = gload4(s.quad, 2 * (size_t)ni);
           qc[q] = make_float2(q1.x, q1.y);
         }
         if (open) {
-          const uint32_t q = sp + 2u * (uint32_t)__popcll(m3 & lt);
+          const uint32_t q = sp + 2u * lanes_below(m3);
           stk[q] = cr;
           stk[q + 1] = cl;  // left child popped first
         }
-        if (bkt) rl[nr + __popcll(m4 & lt)] = make_uint2(nlo, ncount);
+        if (bkt) rl[nr + lanes_below(m4)] = make_uint2(nlo, ncount);
         if (FAR) {
           const unsigned long long m5 = __ballot(far);
-          if (far) fl[nf + __popcll(m5 & lt)] = ni;
+          if (far) fl[nf + lanes_below(m5)] = ni;
           nf += (uint32_t)__popcll(m5);
         }
         np += (uint32_t)__popcll(m1);
