@@ -55,20 +55,20 @@ FLOPS_QUAD = 56
 # operations, FMAs counted twice) and the evaluation of the expansion at one body
 FLOPS_M2L = 230
 FLOPS_L2P = 140
-# HBM bytes of one pool-wide walk launch on this workload, from two rocprofv3 passes of
-# `bench.py --profile --leaf-size 1 --far-theta 0.6 --mono-ratio 0.9 --group-size 64
-# --group-lanes 4 --radix-bits 8 --morton-bits 30` (--pmc FETCH_SIZE, then WRITE_SIZE),
-# scaled by the same counter on a 1 GiB device copy (`bench.py --copy-calibration 1024`):
-# FETCH_SIZE reads half the copied bytes (x2), WRITE_SIZE all of them (x1). 30.76 GB fetched
-# + 0.57 GB written; the tested node records come from beyond L2.
-WALK_TRAFFIC_BYTES = 30.760e9 + 0.569e9
-# (leaf size, far_theta, mono_ratio, group size, lanes per body, key bits) the walk counters were
-# measured on
-WALK_TRAFFIC_CONFIG = (1, 0.6, 0.9, 64, 4, 30)
-# SQ_INSTS_VALU / SQ_BUSY_CU_CYCLES of the walk, same config: VALU wave-instructions per
-# CU-cycle; one per CU-cycle (four SIMDs each taking four cycles per wave64 instruction) is
-# the issue limit, so the walk is VALU-issue bound at whatever clock the GPU runs
-WALK_VALU_PER_CU_CYCLE = 1.00
+# Counters of one pool-wide walk launch on this workload, per tuned configuration (leaf size,
+# far_theta, mono_ratio, group size, lanes per body, key bits) — the two the tuner lands on,
+# a per cent apart: HBM bytes from two rocprofv3 passes of `bench.py --profile --leaf-size 1
+# --far-theta F --mono-ratio M --group-size 64 --group-lanes 4 --radix-bits 8
+# --morton-bits 30` (--pmc FETCH_SIZE, then WRITE_SIZE), scaled by the same counter on a
+# 1 GiB device copy (`bench.py --copy-calibration 1024`): FETCH_SIZE reads half the copied
+# bytes (x2), WRITE_SIZE all of them (x1) — the tested node records come from beyond L2;
+# and SQ_INSTS_VALU / SQ_BUSY_CU_CYCLES, VALU wave-instructions per CU-cycle: one per
+# CU-cycle (four SIMDs each taking four cycles per wave64 instruction) is the issue limit,
+# so the walk is VALU-issue bound at whatever clock the GPU runs.
+WALK_COUNTERS = {
+    (1, 0.6, 0.9, 64, 4, 30): {"traffic": 30.758e9 + 0.569e9, "valu_per_cu_cycle": 0.99},
+    (1, 0.65, 0.8, 64, 4, 30): {"traffic": 33.042e9 + 0.569e9, "valu_per_cu_cycle": 0.99},
+}
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 
 
@@ -704,17 +704,17 @@ def main():
             # 230 per far node, 140 per body of a local expansion)
             "achieved_executed": b["walk"]["tflops"],
             "frac_executed": b["walk"]["tflops"] / FP32_PEAK_TFLOPS,
-            "traffic": WALK_TRAFFIC_BYTES if knobs == WALK_TRAFFIC_CONFIG else None,
+            "traffic": WALK_COUNTERS.get(knobs, {}).get("traffic"),
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2) + WRITE_SIZE (x1), 1 GiB copy scale,"
-                              " leaf size 1, far-field ratio 0.6, monopole band 0.9, groups"
-                              " of 64, 4 lanes, 30-bit keys",
+                              " on the same knobs (measured for the tuner's two usual picks;"
+                              " null for others)",
             "flops_per_launch": b["walk"]["flops_ref"],
             "executed_flops_per_launch": b["walk"]["flops"],
             "walk_ms": b["walk"]["ms"],
             # VALU wave-instructions per CU busy cycle (rocprofv3 SQ_INSTS_VALU /
             # SQ_BUSY_CU_CYCLES, profiles/sq_counters_walk_ensemble512.md): at the issue
             # limit of one per cycle although its counted flops are a fraction of peak
-            "valu_per_cu_cycle": WALK_VALU_PER_CU_CYCLE if knobs == WALK_TRAFFIC_CONFIG else None,
+            "valu_per_cu_cycle": WALK_COUNTERS.get(knobs, {}).get("valu_per_cu_cycle"),
         },
         "batched": strip(b),
         "tuning": tuning,
