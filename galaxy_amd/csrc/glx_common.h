@@ -46,10 +46,10 @@ constexpr int kGroup = 64;                             // targets per group walk
 // sibling pairs.
 struct __align__(16) Node {
   float4 a;  // centre of mass (origin-relative f32), mass
-  float4 h;  // squared acceptance radius (s/theta + delta)^2, children: split | leaf flags
-             // (as bits); then, with quadrupoles, the group walk's squared monopole-band
-             // radius and the node's radius about its com, otherwise the next node after
-             // this subtree (unified or kEnd) and the right child (as bits)
+  float4 h;  // squared acceptance radius (s/theta + delta)^2, children as walk entries
+             // (enc_entries, as bits); then, with quadrupoles, the group walk's squared
+             // monopole-band radius and the node's radius about its com, otherwise the
+             // next node after this subtree (unified or kEnd)
 };
 struct __align__(16) NodeBox {
   float4 lo;  // AABB min (origin-relative f32), first leaf (as bits)
@@ -68,6 +68,31 @@ __host__ __device__ inline uint32_t left_child(uint32_t e, uint32_t n) {
 }
 __host__ __device__ inline uint32_t right_child(uint32_t e, uint32_t n) {
   return (e & kRightLeaf) ? (e & kSplitMask) + 1u : n + (e & kSplitMask) + 1u;
+}
+
+// The group walk addresses a segment's leaves, classifier records and quadrupoles by
+// 32-bit byte offsets from its leaf array when all three lie within 4 GiB of it (the
+// segment layout keeps them adjacent, 80 B per body: up to about 5e7 bodies); other
+// segments take the walk's 64-bit-address form.
+__host__ __device__ inline bool walk_off32(uint64_t leaf, uint64_t node, uint64_t quad, uint32_t n) {
+  const uint64_t lim = 0xFFFFFFFFull, dn = node - leaf, dq = quad - leaf;
+  return dn <= lim && dq <= lim && 16ull * n <= lim && dn + 32ull * n <= lim &&
+         dq + 32ull * n <= lim;
+}
+
+// Children in the node record as walk entries (leaf g -> g, internal node g -> g | 2^31):
+// the left child's entry, with bit 30 set when the right child (g + 1 for a split at g) is
+// of the other kind (n <= 2^29). c is a unified index (leaf c < n, internal node c - n).
+constexpr uint32_t kEntryNode = 1u << 31, kEntryFlip = 1u << 30;
+__host__ __device__ inline uint32_t walk_entry(uint32_t c, uint32_t n) {
+  return c < n ? c : (c - n) | kEntryNode;
+}
+__host__ __device__ inline uint32_t enc_entries(uint32_t cl, uint32_t cr, uint32_t n) {
+  return walk_entry(cl, n) | ((cl < n) != (cr < n) ? kEntryFlip : 0u);
+}
+__host__ __device__ inline uint32_t left_entry(uint32_t w) { return w & ~kEntryFlip; }
+__host__ __device__ inline uint32_t right_entry(uint32_t w) {
+  return (w & ~kEntryFlip) + 1u + ((w & kEntryFlip) << 1);  // the flip toggles the kind bit
 }
 
 // Build/walk configuration shared by all segments of a launch sequence.

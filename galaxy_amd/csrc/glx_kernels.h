@@ -43,8 +43,11 @@ hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, co
 // systems larger than one super-tile (none: nmt = 0).
 hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs, uint32_t nseg,
                             const Tile* at, uint32_t nat, const Tile* mt, uint32_t nmt);
+// wide: some segment's leaves, node records or quadrupoles lie beyond 4 GiB of its leaf
+// array (walk_off32 false), so the group walk takes its 64-bit-address form.
 hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* wt,
-                       uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count);
+                       uint32_t nwt, const Tile* gt, uint32_t ngt, int mode, bool count,
+                       bool wide);
 hipError_t launch_diag(hipStream_t st, const SegDesc* segs, uint32_t nseg, const Tile* et,
                        uint32_t net, double* partial, double* out);
 // FP64 direct sum onto nt targets: the sources split into `chunks` ranges of chunk_len

@@ -111,6 +111,7 @@ class Batch {
   Tile* wtiles_ = nullptr;  // kBlock targets per tile (per-target walk)
   Tile* gtiles_ = nullptr;  // kGroup targets per tile (group walk)
   uint32_t n_et_ = 0, n_at_ = 0, n_mt_ = 0, n_ct_ = 0, n_wt_ = 0, n_gt_ = 0;
+  bool wide_ = false;  // a segment needs the group walk's 64-bit-address form
   uint32_t* tile_hist_ = nullptr;
   uint32_t* tile_off_ = nullptr;
   double* diag_partial_ = nullptr;

@@ -1,6 +1,7 @@
 // Host runtime: device segments, batches and launch sequences.
 #include "glx_runtime.h"
 
+#include <cstdlib>
 #include <mutex>
 
 #include "galaxy_lbvh.h"
@@ -88,10 +89,12 @@ Layout layout_for(uint32_t n, bool with_acc_out) {
   L.k1 = c.take<uint64_t>(N);
   L.v0 = c.take<uint32_t>(N);
   L.v1 = c.take<uint32_t>(N);
+  // leaves, classifier records and quadrupoles adjacent: the group walk addresses all
+  // three with 32-bit byte offsets from the leaf array (80 B per body)
   L.leaf = c.take<float4>(N);
   L.node = c.take<Node>(N);
-  L.box = c.take<NodeBox>(N);
   L.quad = c.take<float4>(2 * N);
+  L.box = c.take<NodeBox>(N);
   L.topo = c.take<uint4>(N);
   L.pleaf = c.take<uint32_t>(N);
   L.rs = c.take<uint32_t>(N);
@@ -222,8 +225,13 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   host_ = descs;
   nseg_ = (uint32_t)descs.size();
   std::vector<Tile> et, at, mt, ct, wt, gt;
+  // GLX_WALK_WIDE=1 takes the 64-bit-address walk for every segment (the two forms give
+  // the same results bit for bit; the tests compare them on systems of any size)
+  const char* force_wide = std::getenv("GLX_WALK_WIDE");
+  wide_ = force_wide && force_wide[0] == '1';
   for (uint32_t s = 0; s < nseg_; ++s) {
     SegDesc& d = host_[s];
+    if (!walk_off32((uint64_t)d.leaf, (uint64_t)d.node, (uint64_t)d.quad, d.n)) wide_ = true;
     d.sort_tile0 = (uint32_t)et.size();
     push_tiles(et, s, d.n, kSortTile);
     d.sort_tiles = (uint32_t)et.size() - d.sort_tile0;
@@ -342,7 +350,8 @@ void Batch::flatten(const FlatOut& out) {
   check(launch_flatten(stream_, segs_, etiles_, n_et_, out), "flatten");
 }
 void Batch::walk(int mode, bool count) {
-  check(launch_walk(stream_, mode_, segs_, wtiles_, n_wt_, gtiles_, n_gt_, mode, count), "walk");
+  check(launch_walk(stream_, mode_, segs_, wtiles_, n_wt_, gtiles_, n_gt_, mode, count, wide_),
+        "walk");
 }
 
 void Batch::build_and_walk(int mode, bool count) {

@@ -197,6 +197,36 @@ def test_pool_radix_width_bit_identical(mode):
             assert np.array_equal(pa, pb) and np.array_equal(va, vb)
 
 
+@pytest.mark.parametrize("precision", ["f32", "kahan"])
+def test_walk_address_forms_bit_identical(precision, monkeypatch):
+    """The group walk addresses leaves, node records and quadrupoles by 32-bit byte offsets
+    from the leaf array; a system whose arrays span more than 4 GiB (about 5e7 bodies) takes
+    the walk's 64-bit-address form, which GLX_WALK_WIDE=1 forces for every system. Both
+    forms step a batch of uneven systems bit-identically (far field and monopole band on)
+    and count the same terms."""
+    sy = systems([40000, 3, 2049, 7000, 65], seed=84)
+    prec = glx.PREC_KAHAN if precision == "kahan" else glx.PREC_F32
+    out = []
+    for wide in ("0", "1"):
+        say("walk address form", precision, "wide" if wide == "1" else "compact")
+        monkeypatch.setenv("GLX_WALK_WIDE", wide)
+        p = Pool(GB, morton_bits=30, walk=glx.WALK_GROUP, multipole=2, far_theta=0.6,
+                 mono_ratio=0.9, precision=prec)
+        ids = [p.admit(pos, vel, mass, softening=0.01, theta=0.5, dt=1e-3)
+               for pos, vel, mass in sy]
+        for _ in range(3):
+            p.step()
+        c = p.count()
+        out.append(([p.snapshot(j) for j in ids], [p.snapshot_accel(j) for j in ids], c))
+        p.close()
+    (sa, aa, ca), (sb, ab, cb) = out
+    for (pa, va), (pb, vb) in zip(sa, sb):
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    for x, y in zip(aa, ab):
+        assert np.array_equal(x, y)
+    assert ca == cb and ca["interactions"] > 0
+
+
 def test_autotuner_block_shape_and_radix_width():
     """The staged tuner picks a block shape within both error bounds and the radix width
     with the fastest sort phase, and its cache file returns the same knobs."""
