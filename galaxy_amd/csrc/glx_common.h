@@ -41,7 +41,7 @@ constexpr int kGroup = 64;                             // targets per group walk
 // Walk record of one internal node, split by use into two arrays of 32-B records: the
 // classifier record, which every node test of the group walk loads, and the box record,
 // which only the tests that need the node's box load (the per-target walk's inside test,
-// the group walk's disjoint-box test at theta > 0.55 and its bucket test). Siblings are
+// the group walk's disjoint-box test at theta > 1 and its bucket test). Siblings are
 // adjacent (children g and g + 1), so one 128-B line holds the classifier records of two
 // sibling pairs.
 struct __align__(16) Node {

@@ -1652,8 +1652,9 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const float eps2 = s.eps2;
     const bool soft = eps2 >= kR2Floor;  // r^2 >= eps^2 needs no floor
     const uint32_t bucket = s.leaf_size;
-    // the disjoint-box test is needed (theta >= 0: compared as bits, a scalar test)
-    const bool loose_theta = __float_as_uint(s.theta) > __float_as_uint(0.55f);
+    // the disjoint-box test is needed only above theta = 1 (theta >= 0: compared as bits,
+    // a scalar test)
+    const bool loose_theta = __float_as_uint(s.theta) > __float_as_uint(1.0f);
     const bool need_box = loose_theta || bucket > 1u;  // box records (bounds, leaf range)
     const uint64_t leaf_base = (uint64_t)s.leaf, node_base = (uint64_t)s.node;
     // The compact form addresses leaves, classifier records and quadrupoles by 32-bit
@@ -1760,9 +1761,11 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           if (BOX) {
             const float4* brec = reinterpret_cast<const float4*>(s.box);
             const float4 B = gload4(brec, nr2), Cc = gload4(brec, nr2 + 1);
-            // Every point of the node's box lies within sqrt(3) s of its com, and an accepted
-            // com is at least s / theta + delta from the group box, so for theta <= 0.55
-            // (1 / theta > sqrt(3) with a margin) acceptance implies disjoint boxes.
+            // Every point of the node's box lies within sqrt(3) s / 2 + delta of its com
+            // (half its diagonal from the box centre, delta from there to the com), and an
+            // accepted com is at least s / theta + delta from the group box, so for theta <= 1
+            // (1 / theta > sqrt(3) / 2 with a margin) acceptance implies disjoint boxes. (A
+            // node of size 0 is one point, whose monopole is exact for every body.)
             if (loose_theta)
               apart = (Cc.x < gx0) | (B.x > gx1) | (Cc.y < gy0) | (B.y > gy1) | (Cc.z < gz0) |
                       (B.z > gz1);
