@@ -297,6 +297,31 @@ def test_far_field_is_third_order():
     assert errs[0] / errs[1] > 14.0
 
 
+@pytest.mark.parametrize("mode", ["monopole", "quadrupole"])
+def test_group_walk_above_theta_one_tests_disjoint_boxes(mode):
+    """Up to theta = 1 acceptance implies that a node's box and the group's box are
+    disjoint (tests/test_walk_acceptance.py), so the group walk reads no box records there;
+    above it the walk also requires disjoint boxes, from the box record. At theta = 1.2, on
+    a Plummer sphere with 200 coincident pairs, the group walk's error vs FP64 stays below
+    the reference per-target walk's at the same theta, and every force is finite."""
+    pos, _, mass = ic.plummer(20000, 31)
+    pos = np.concatenate([pos, pos[:200]])
+    mass = np.concatenate([mass, mass[:200]])
+    tg = np.arange(0, pos.shape[0], 37, dtype=np.uint32)
+    d64 = glx.direct_sum_f64(pos, mass, 1.0, 0.01, tg)
+
+    def err(solver):
+        a = solver.accelerations(pos, mass)
+        assert np.isfinite(a).all()
+        e = np.linalg.norm(a[tg] - d64, axis=1) / np.linalg.norm(d64, axis=1)
+        return float(np.median(e)), float(np.quantile(e, 0.99))
+
+    ref = err(glx.Lbvh(1.0, 0.01, 1.2))
+    grp = err(glx.Lbvh(1.0, 0.01, 1.2, **(GROUP_MONO if mode == "monopole" else FAST)))
+    say(f"theta 1.2 {mode}: group {grp}, reference walk {ref}")
+    assert grp[0] <= ref[0] and grp[1] <= ref[1]
+
+
 def test_monopole_band_limits():
     """mono_ratio -> 0 keeps every quadrupole (bit-equal to the band off); mono_ratio -> 1
     turns (nearly) every accepted node into a monopole, i.e. the group walk without
