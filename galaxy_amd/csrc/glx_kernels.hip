@@ -186,6 +186,32 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gblw(T* p) {
   return (__attribute__((address_space(1))) T*)p;
 }
 
+// MI355X deals the blocks of a launch round-robin over its 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2. Walk groups that are neighbours in Morton order read
+// nearly the same nodes, so the walks renumber blocks: in every span of 8 * kXcdRun
+// blocks, the kXcdRun blocks an XCD receives take kXcdRun consecutive tiles. Neighbours
+// then share an L2 while all XCDs still advance through the tiles together (a whole-range
+// split per XCD leaves XCDs with unequal work). Launches use a grid that is a multiple of
+// the span (xcd_grid); any other grid keeps the hardware order.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdRun = 8;
+constexpr uint32_t kXcdSpan = kXcds * kXcdRun;
+
+template <uint32_t RUN = kXcdRun>
+__device__ __forceinline__ uint32_t xcd_block() {
+  constexpr uint32_t span = kXcds * RUN;
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  if (g % span) return b;
+  const uint32_t w = b % span;
+  return (b - w) + (w % kXcds) * RUN + w / kXcds;
+}
+
+// The build kernels over sort tiles (2048 keys) take runs of 32 tiles per XCD, so that a
+// system of up to 65,536 bodies is gathered, sorted and linked on one XCD and its random
+// reads (positions by sorted index, codes in the binary searches) and scattered writes
+// stay in that XCD's L2.
+constexpr uint32_t kXcdBuildRun = 32;
+
 // ----------------------------------------------------------------------------------
 // 1. bounding box
 // ----------------------------------------------------------------------------------
@@ -346,7 +372,7 @@ k_radix_upsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles
                 uint32_t* __restrict__ tile_hist, int shift, int src) {
   constexpr int BINS = 1 << B;
   __shared__ uint32_t hist[BINS];
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block<kXcdBuildRun>(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const auto* keys = gbl(static_cast<const K*>(s.keys[src]) + tl.start);
@@ -477,7 +503,7 @@ k_radix_downsweep(const SegDesc* __restrict__ segs, const Tile* __restrict__ til
   static_assert(R * kSortThreads >= BINS, "the tile scan covers every digit");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block<kXcdBuildRun>(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const auto* keys_in = gbl(static_cast<const K*>(s.keys[src]) + tl.start);
@@ -571,7 +597,7 @@ constexpr int kGatherThreads = 1024;
 
 __global__ void __launch_bounds__(kGatherThreads)
 k_gather(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block<kXcdBuildRun>(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const double o0 = s.bounds[5], o1 = s.bounds[6], o2 = s.bounds[7];
@@ -632,7 +658,7 @@ template <class K>
 __global__ void __launch_bounds__(kKarrasThreads)
 k_karras(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles) {
   __shared__ K win[kSortTile + 2 * kKarrasPad];
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block<kXcdBuildRun>(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     const int n = (int)s.n;
@@ -1253,24 +1279,6 @@ k_flatten(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint
 //    index and, in resident mode, applies the closing half-kick
 //    (core/src/integrator.rs:70-73).
 // ----------------------------------------------------------------------------------
-
-// MI355X deals the blocks of a launch round-robin over its 8 XCDs (block b runs on XCD
-// b % 8), each with its own L2. Walk groups that are neighbours in Morton order read
-// nearly the same nodes, so the walks renumber blocks: in every span of 8 * kXcdRun
-// blocks, the kXcdRun blocks an XCD receives take kXcdRun consecutive tiles. Neighbours
-// then share an L2 while all XCDs still advance through the tiles together (a whole-range
-// split per XCD leaves XCDs with unequal work). Launches use a grid that is a multiple of
-// the span (xcd_grid); any other grid keeps the hardware order.
-constexpr uint32_t kXcds = 8;
-constexpr uint32_t kXcdRun = 8;
-constexpr uint32_t kXcdSpan = kXcds * kXcdRun;
-
-__device__ __forceinline__ uint32_t xcd_block() {
-  const uint32_t g = gridDim.x, b = blockIdx.x;
-  if (g % kXcdSpan) return b;
-  const uint32_t w = b % kXcdSpan;
-  return (b - w) + (w % kXcds) * kXcdRun + w / kXcds;
-}
 
 struct Acc3 {
   float x, y, z, cx, cy, cz;
@@ -2137,11 +2145,13 @@ static inline uint32_t grid_for(uint32_t ntiles, uint32_t cap = 65535u) {
   return ntiles == 0 ? 1u : (ntiles < cap ? ntiles : cap);
 }
 
-// Grid for the XCD-ordered walks: a multiple of the span (idle blocks skip the tile loop).
+// Grid for the XCD-ordered kernels: a multiple of the span (idle blocks skip the tile loop).
+template <uint32_t RUN = kXcdRun>
 static inline uint32_t xcd_grid(uint32_t ntiles) {
-  if (ntiles < kXcdSpan) return ntiles == 0 ? 1u : ntiles;
-  const uint32_t g = (ntiles + kXcdSpan - 1) / kXcdSpan * kXcdSpan;
-  return g < 65472u ? g : 65472u;
+  constexpr uint32_t span = kXcds * RUN, cap = 65535u / span * span;
+  if (ntiles < span) return ntiles == 0 ? 1u : ntiles;
+  const uint32_t g = (ntiles + span - 1) / span * span;
+  return g < cap ? g : cap;
 }
 
 hipError_t launch_seg_reset(hipStream_t st, const SegDesc* segs, uint32_t nseg) {
@@ -2196,8 +2206,8 @@ static void sort_passes(hipStream_t st, const SegDesc* segs, uint32_t nseg, cons
   int src = sort_start(key63, B);
   for (int pass = 0; pass < sort_passes_for(key63, B); ++pass) {
     const int shift = pass * B;
-    hipLaunchKernelGGL((k_radix_upsweep<K, B>), dim3(grid_for(net)), dim3(kSortThreads), 0, st,
-                       segs, et, net, tile_hist, shift, src);
+    hipLaunchKernelGGL((k_radix_upsweep<K, B>), dim3(xcd_grid<kXcdBuildRun>(net)),
+                       dim3(kSortThreads), 0, st, segs, et, net, tile_hist, shift, src);
     if (sc.nchunks)
       hipLaunchKernelGGL(k_radix_chunk_sum<B>, dim3(grid_for(sc.nchunks)), dim3(kChunkThreads), 0,
                          st, sc.chunks, sc.nchunks, tile_hist, sc.chunk_rows);
@@ -2206,8 +2216,8 @@ static void sort_passes(hipStream_t st, const SegDesc* segs, uint32_t nseg, cons
     if (sc.nchunks)
       hipLaunchKernelGGL(k_radix_chunk_offsets<B>, dim3(grid_for(sc.nchunks)), dim3(kChunkThreads),
                          0, st, sc.chunks, sc.nchunks, tile_hist, sc.chunk_rows, tile_off);
-    hipLaunchKernelGGL((k_radix_downsweep<K, B>), dim3(grid_for(net)), dim3(kSortThreads), 0, st,
-                       segs, et, net, tile_off, shift, src);
+    hipLaunchKernelGGL((k_radix_downsweep<K, B>), dim3(xcd_grid<kXcdBuildRun>(net)),
+                       dim3(kSortThreads), 0, st, segs, et, net, tile_off, shift, src);
     src ^= 1;
   }
 }
@@ -2234,13 +2244,15 @@ hipError_t launch_sort(hipStream_t st, const Mode& md, const SegDesc* segs, uint
 }
 
 hipError_t launch_gather(hipStream_t st, const SegDesc* segs, const Tile* et, uint32_t net) {
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kGatherThreads), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather, dim3(xcd_grid<kXcdBuildRun>(net)), dim3(kGatherThreads), 0, st,
+                     segs, et, net);
   return hipGetLastError();
 }
 
 hipError_t launch_refresh_leaves(hipStream_t st, const SegDesc* segs, const Tile* et,
                                  uint32_t net) {
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(net)), dim3(kGatherThreads), 0, st, segs, et, net);
+  hipLaunchKernelGGL(k_gather, dim3(xcd_grid<kXcdBuildRun>(net)), dim3(kGatherThreads), 0, st,
+                     segs, et, net);
   return hipGetLastError();
 }
 
@@ -2253,11 +2265,11 @@ hipError_t launch_flatten(hipStream_t st, const SegDesc* segs, const Tile* et, u
 hipError_t launch_karras(hipStream_t st, const Mode& md, const SegDesc* segs, const Tile* et,
                          uint32_t net) {
   if (md.key63)
-    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(grid_for(net)), dim3(kKarrasThreads), 0, st, segs,
-                       et, net);
+    hipLaunchKernelGGL(k_karras<uint64_t>, dim3(xcd_grid<kXcdBuildRun>(net)), dim3(kKarrasThreads),
+                       0, st, segs, et, net);
   else
-    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(grid_for(net)), dim3(kKarrasThreads), 0, st, segs,
-                       et, net);
+    hipLaunchKernelGGL(k_karras<uint32_t>, dim3(xcd_grid<kXcdBuildRun>(net)), dim3(kKarrasThreads),
+                       0, st, segs, et, net);
   return hipGetLastError();
 }
 
