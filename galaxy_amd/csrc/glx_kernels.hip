@@ -214,6 +214,11 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // stay in that XCD's L2.
 constexpr uint32_t kXcdBuildRun = 32;
 
+// The per-target walk keeps runs of 8 tiles: its cost per target varies far more across a
+// system (dense cores against halos), and long runs left whole cores to one XCD (2^20
+// collision, reference build: 1.8x slower at 256).
+constexpr uint32_t kXcdLaneRun = 8;
+
 // ----------------------------------------------------------------------------------
 // 1. bounding box
 // ----------------------------------------------------------------------------------
@@ -1396,7 +1401,7 @@ template <bool KAHAN, bool COUNT>
 __global__ void __launch_bounds__(kBlock)
 k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, uint32_t ntiles,
             int mode) {
-  for (uint32_t t = xcd_block(); t < ntiles; t += gridDim.x) {
+  for (uint32_t t = xcd_block<kXcdLaneRun>(); t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
     const SegDesc& s = segs[tl.seg];
     if (tl.start == 0 && threadIdx.x < 6) bbox_neutral(s.bbox, threadIdx.x);
@@ -2326,7 +2331,7 @@ hipError_t launch_walk(hipStream_t st, const Mode& md, const SegDesc* segs, cons
     }
     return hipGetLastError();
   }
-  const dim3 g(xcd_grid(nwt)), b(kBlock);
+  const dim3 g(xcd_grid<kXcdLaneRun>(nwt)), b(kBlock);
   if (md.kahan) {
     if (count)
       hipLaunchKernelGGL((k_walk_lane<true, true>), g, b, 0, st, segs, wt, nwt, mode);
