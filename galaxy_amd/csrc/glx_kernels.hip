@@ -2154,8 +2154,8 @@ static inline uint32_t grid_for(uint32_t ntiles, uint32_t cap = 65535u) {
 
 // Grid for the XCD-ordered kernels: a multiple of the span (idle blocks skip the tile loop).
 template <uint32_t RUN = kXcdRun>
-static inline uint32_t xcd_grid(uint32_t ntiles) {
-  constexpr uint32_t span = kXcds * RUN, cap = 65535u / span * span;
+static inline uint32_t xcd_grid(uint32_t ntiles, uint32_t max_blocks = 65535u) {
+  const uint32_t span = kXcds * RUN, cap = max_blocks / span * span;
   if (ntiles < span) return ntiles == 0 ? 1u : ntiles;
   const uint32_t g = (ntiles + span - 1) / span * span;
   return g < cap ? g : cap;
@@ -2299,7 +2299,9 @@ hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs,
 template <bool KAHAN, bool COUNT, bool WIDE>
 static void walk_group_q(hipStream_t st, bool quad, bool far, const SegDesc* segs, const Tile* gt,
                          uint32_t ngt, int mode) {
-  const dim3 g(xcd_grid(ngt)), b(kGroup);
+  // one block per walk tile (up to 2^25 blocks of one wave): a block that took a second
+  // tile a grid's length later would mix two distant regions in its XCD's L2
+  const dim3 g(xcd_grid(ngt, 1u << 25)), b(kGroup);
   if (quad && far)
     hipLaunchKernelGGL((k_walk_group<KAHAN, COUNT, true, true, WIDE>), g, b, 0, st, segs, gt, ngt,
                        mode);
