@@ -169,6 +169,7 @@ struct SegDesc {
   uint32_t leaf_size;  // group walk: nodes of at most this many bodies are summed body by body
   uint32_t group_size;  // group walk: bodies per walk group (subtrees of at most this many)
   uint32_t group_lanes; // group walk: most lanes per body in groups of at most 32 bodies
+  uint32_t stack_cap;   // group walk: stack entries before the per-target fallback (0: all)
   float g, eps2, theta, half_dt;
   float far2;  // squared far-field ratio: (node radius + group radius)^2 <= far2 * R^2
   float mono_inv;  // 1 / (theta mono_ratio): accepted nodes this far are monopoles (0: off)

@@ -1337,6 +1337,9 @@ template <bool KAHAN, bool COUNT>
 __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 me, Acc3& a,
                                           unsigned long long& n_int,
                                           unsigned long long& n_vis, bool quad_records) {
+  // no contraction: the same bits whichever kernel inlines it (the per-target walk, or the
+  // group walk's fallback in either record layout)
+#pragma clang fp contract(off)
   const uint32_t n = s.n;
   const float eps2 = s.eps2;
   const float4* __restrict__ leaf = s.leaf;
@@ -1678,6 +1681,9 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
     const bool fits = WIDE || walk_off32(leaf_base, node_base, (uint64_t)s.quad, n);
     const char* const lbase = reinterpret_cast<const char*>(s.leaf);
     const char* const lbase16 = lbase + 16;  // second float4 of a record
+    // stack entries before a group falls back to the per-target walk
+    const uint32_t scap = s.stack_cap && s.stack_cap < (uint32_t)kStackCap ? s.stack_cap
+                                                                           : (uint32_t)kStackCap;
     const uint32_t dn32 = (uint32_t)(node_base - leaf_base),
                    dq32 = (uint32_t)((uint64_t)s.quad - leaf_base);
     // the walk groups (subtrees of at most kGroup bodies, marked by the tree build)
@@ -1818,7 +1824,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
           unsigned long long m4 = 0ull;
           if (BOX) m4 = __ballot(bkt);
           const uint32_t c3 = (uint32_t)__popcll(m3);
-          if (sp + 2u * c3 > (uint32_t)kStackCap) {
+          if (sp + 2u * c3 > scap) {
             overflow = true;
             break;
           }

@@ -229,8 +229,13 @@ void Batch::set_segments(const std::vector<SegDesc>& descs) {
   // the same results bit for bit; the tests compare them on systems of any size)
   const char* force_wide = std::getenv("GLX_WALK_WIDE");
   wide_ = force_wide && force_wide[0] == '1';
+  // GLX_WALK_STACK_CAP=k caps the group walk's stack at k entries, so that the tests can
+  // drive groups into the per-target fallback (default: the full LDS stack)
+  const char* cap = std::getenv("GLX_WALK_STACK_CAP");
+  const uint32_t stack_cap = cap ? (uint32_t)std::strtoul(cap, nullptr, 10) : 0u;
   for (uint32_t s = 0; s < nseg_; ++s) {
     SegDesc& d = host_[s];
+    d.stack_cap = stack_cap;
     if (!walk_off32((uint64_t)d.leaf, (uint64_t)d.node, (uint64_t)d.quad, d.n)) wide_ = true;
     d.sort_tile0 = (uint32_t)et.size();
     push_tiles(et, s, d.n, kSortTile);

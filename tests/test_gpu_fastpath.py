@@ -322,6 +322,28 @@ def test_group_walk_above_theta_one_tests_disjoint_boxes(mode):
     assert grp[0] <= ref[0] and grp[1] <= ref[1]
 
 
+@pytest.mark.parametrize("multipole", [0, 2])
+def test_stack_overflow_falls_back_to_the_per_target_walk(monkeypatch, multipole):
+    """A group whose LDS stack would overflow is walked per target instead (the reference
+    criterion and monopoles, the lane walk of the reference build). GLX_WALK_STACK_CAP=1
+    sends every group there: with 30-bit keys (the reference's tree) the forces then equal
+    the reference build's bit for bit, in either node-record layout, the far field and the
+    monopole band taking no part, and every group is counted as a fallback."""
+    pos, _, mass = ic.plummer(20000, 41)
+    ref = glx.Lbvh(1.0, 0.01, 0.5, morton_bits=30, walk=glx.WALK_REFERENCE,
+                   multipole=0).accelerations(pos, mass)
+    monkeypatch.setenv("GLX_WALK_STACK_CAP", "1")
+    knobs = dict(far_theta=0.6, mono_ratio=0.9) if multipole == 2 else {}
+    s = glx.Lbvh(1.0, 0.01, 0.5, count=True, morton_bits=30, walk=glx.WALK_GROUP,
+                 multipole=multipole, **knobs)
+    a = s.accelerations(pos, mass)
+    st = s.stats()
+    say("stack cap 1, multipole", multipole, ": fallbacks", st["fallbacks"], "far nodes",
+        st["far_nodes"])
+    assert st["fallbacks"] >= 20000 // 64 and st["far_nodes"] == 0
+    assert np.array_equal(a, ref)
+
+
 def test_monopole_band_limits():
     """mono_ratio -> 0 keeps every quadrupole (bit-equal to the band off); mono_ratio -> 1
     turns (nearly) every accepted node into a monopole, i.e. the group walk without
