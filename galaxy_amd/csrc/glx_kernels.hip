@@ -820,7 +820,8 @@ __device__ __forceinline__ AggV fold_agg(const AggV& l, const AggV& r) {
 template <bool QUAD>
 __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const AggV& r,
                                            float inv_theta, uint32_t lo, uint32_t hi,
-                                           uint32_t cl, uint32_t cr, uint32_t nx) {
+                                           uint32_t cl, uint32_t cr, uint32_t nx,
+                                           bool write_box = true) {
 #pragma clang fp contract(off)
   const float cx = (r.mn.x + r.mx.x) * 0.5f, cy = (r.mn.y + r.mx.y) * 0.5f,
               cz = (r.mn.z + r.mx.z) * 0.5f;
@@ -850,7 +851,7 @@ __device__ __forceinline__ void store_node(const SegDesc& s, uint32_t j, const A
     nd.h = make_float4(rcrit * rcrit, ch, __uint_as_float(nx), 0.f);
   }
   s.node[j] = nd;
-  s.box[j] = bx;
+  if (write_box) s.box[j] = bx;
   if (QUAD) {
     s.quad[2 * j] = make_float4(r.q[0], r.q[1], r.q[2], r.q[3]);
     s.quad[2 * j + 1] = make_float4(r.q[4], r.q[5], 0.f, 0.f);
@@ -927,7 +928,7 @@ __device__ __forceinline__ AggV lds_child(const float4 (*agg)[kAggTile], uint32_
 template <bool QUAD>
 __global__ void __launch_bounds__(kAggThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles,
-                 uint32_t ntiles) {
+                 uint32_t ntiles, int group) {
   constexpr int NP = QUAD ? 4 : 3;  // com+mass, min(+q0), max(+q1), q2..q5
   __shared__ float4 s_agg[NP][kAggTile];
   __shared__ uint2 s_info[kAggTile];
@@ -942,6 +943,10 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
     if (n < 2) continue;
     const uint32_t t0 = tl.start, cnt = tl.count, t1 = t0 + cnt;
     const float inv_theta = 1.0f / s.theta;
+    // every node's box record is read by the per-target walk and by the group walk above
+    // theta = 1 (compared as bits, as the walk does) or with buckets
+    const bool box_all = !group || __float_as_uint(s.theta) > __float_as_uint(1.0f) ||
+                         s.leaf_size > 1u;
     if (e < kMaxHeight) s_hcur[e] = 0u;
     if (e == 0) s_nroots = 0u;
     float4 lf = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1033,7 +1038,12 @@ k_aggregate_tile(const SegDesc* __restrict__ segs, const Tile* __restrict__ tile
         const uint32_t lo = t0 + (inf.x & 1023u), hi = t0 + ((inf.x >> 10) & 1023u);
         const uint32_t cl = ll ? t0 + g : n + t0 + g, cr = rl ? t0 + g + 1 : n + t0 + g + 1;
         const uint32_t nx = QUAD ? 0u : (hi == n - 1 ? kEnd : gbl(s.rs)[hi]);
-        store_node<QUAD>(s, t0 + q, r, inv_theta, lo, hi, cl, cr, nx);
+        // box records: every node's for the builds that read them (per-target walk, the
+        // group walk above theta = 1 or with buckets); otherwise only those of the roots
+        // handed to the climbing passes (no in-tile parent), which fold their boxes
+        const bool in_parent = (inf.x & kInParent) &&
+                               (s_info[(inf.x >> 21) & 1023u].x & kInInside);
+        store_node<QUAD>(s, t0 + q, r, inv_theta, lo, hi, cl, cr, nx, box_all || !in_parent);
       }
       lds_barrier();
     }
@@ -1336,7 +1346,8 @@ __device__ __forceinline__ void store_acc(const SegDesc& s, uint32_t k, const Ac
 template <bool KAHAN, bool COUNT>
 __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 me, Acc3& a,
                                           unsigned long long& n_int,
-                                          unsigned long long& n_vis, bool quad_records) {
+                                          unsigned long long& n_vis, bool quad_records,
+                                          bool boxes) {
   // no contraction: the same bits whichever kernel inlines it (the per-target walk, or the
   // group walk's fallback in either record layout)
 #pragma clang fp contract(off)
@@ -1366,10 +1377,10 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
       const Node* nd = node + (u - n);
       const float4 A = nd->a, H = nd->h;
       // quadrupole builds keep classifier radii in h.zw: the skip pointer is rs[hi], with
-      // hi from the box record (loaded here only then: the group walk's rare fallback)
+      // hi from the topology record (loaded here only then: the group walk's rare fallback)
       uint32_t nxt = __float_as_uint(H.z);
       if (quad_records) {
-        const uint32_t hi = __float_as_uint(s.box[u - n].hi.w);
+        const uint32_t hi = s.topo[u - n].y;
         nxt = hi == n - 1 ? kEnd : rs[hi];
       }
       if (!(A.w > 0.0f)) {
@@ -1378,9 +1389,11 @@ __device__ __forceinline__ void lane_walk(const SegDesc& s, uint32_t k, float4 m
       }
       const float dx = A.x - me.x, dy = A.y - me.y, dz = A.z - me.z;
       const float d2 = dx * dx + dy * dy + dz * dz;
-      // the box is read only for a node far enough to be accepted (the inside test)
+      // the box is read only for a node far enough to be accepted (the inside test). A
+      // build without every box record has theta <= 1, where a target inside a node's box
+      // is never accepted: it lies within sqrt(3) s / 2 + delta < s / theta + delta of the com
       bool accept = d2 >= H.x;
-      if (accept) {
+      if (accept && boxes) {
         const float4 B = s.box[u - n].lo, Cc = s.box[u - n].hi;
         accept = !((B.x <= me.x) & (me.x <= Cc.x) & (B.y <= me.y) & (me.y <= Cc.y) &
                    (B.z <= me.z) & (me.z <= Cc.z));
@@ -1413,7 +1426,7 @@ k_walk_lane(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, ui
     if (s.active && !s.active[min(s.vals[0][k], s.n - 1)]) continue;
     Acc3 a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     unsigned long long n_int = 0, n_vis = 0;
-    lane_walk<KAHAN, COUNT>(s, k, s.leaf[k], a, n_int, n_vis, false);
+    lane_walk<KAHAN, COUNT>(s, k, s.leaf[k], a, n_int, n_vis, false, true);
     store_acc<KAHAN>(s, k, a, mode);
     if (COUNT) {
       atomicAdd(&s.counters[kCntInteractions], n_int);
@@ -1916,7 +1929,7 @@ k_walk_group(const SegDesc* __restrict__ segs, const Tile* __restrict__ tiles, u
         a = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         n_int = 0;
         n_quad = 0;
-        if (owner) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis, QUAD);
+        if (owner) lane_walk<KAHAN, COUNT>(s, k, me, a, n_int, n_vis, QUAD, need_box);
         if (lane == 0) atomicAdd(&s.counters[kCntOverflow], 1ull);
       }  // otherwise the lists were drained in the last round
       if (rep > 1u) {  // fold each body's lanes into its owner (lanes lane + j gcount < 64)
@@ -2291,11 +2304,11 @@ hipError_t launch_aggregate(hipStream_t st, const Mode& md, const SegDesc* segs,
   const dim3 tg(grid_for(nat)), tb(kAggThreads), sg(grid_for(nseg)), sb(kTopThreads);
   const dim3 mg(grid_for(nmt));
   if (md.quad) {
-    hipLaunchKernelGGL(k_aggregate_tile<true>, tg, tb, 0, st, segs, at, nat);
+    hipLaunchKernelGGL(k_aggregate_tile<true>, tg, tb, 0, st, segs, at, nat, (int)md.group);
     if (nmt) hipLaunchKernelGGL(k_aggregate_mid<true>, mg, sb, 0, st, segs, mt, nmt);
     hipLaunchKernelGGL(k_aggregate_top<true>, sg, sb, 0, st, segs, nseg);
   } else {
-    hipLaunchKernelGGL(k_aggregate_tile<false>, tg, tb, 0, st, segs, at, nat);
+    hipLaunchKernelGGL(k_aggregate_tile<false>, tg, tb, 0, st, segs, at, nat, (int)md.group);
     if (nmt) hipLaunchKernelGGL(k_aggregate_mid<false>, mg, sb, 0, st, segs, mt, nmt);
     hipLaunchKernelGGL(k_aggregate_top<false>, sg, sb, 0, st, segs, nseg);
   }
