@@ -72,18 +72,27 @@ WALK_COUNTERS = {
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 
 
-def phase_bytes(morton_bits, radix_bits, quad):
+def phase_bytes(morton_bits, radix_bits, quad, boxes=True):
     """Algorithmic HBM bytes per particle of each HBM-bound phase: kick-drift reads v and a
     (f32) and x (f64) and writes v and x; encode reads x and writes key + payload; each of
     the ceil(bits / radix_bits) LSD sort passes reads the keys for its histogram, then reads
     and writes key + payload; the gather reads the order, x and m (f64) and writes a 16-B
     leaf; Karras reads the codes and writes the 16-B topology record, a leaf parent and a
     skip entry; the bottom-up pass reads the leaf, its parent and the topology and writes
-    the 32-B classifier and 32-B box records (and the 32-B quadrupole)."""
+    the 32-B classifier record, the 32-B box record (boxes: only the builds that read it;
+    group builds at theta <= 1 with leaf size 1 write it for a few handed-over roots only)
+    and the 32-B quadrupole."""
     kb = 4 if morton_bits == 30 else 8
     passes = -(-(30 if morton_bits == 30 else 63) // (radix_bits or 8))
     return {"kick_drift_bbox": 84, "encode": 28 + kb, "sort": (3 * kb + 8) * passes,
-            "gather": 52, "karras": 24 + kb, "aggregate": 132 if quad else 100}
+            "gather": 52, "karras": 24 + kb,
+            "aggregate": 68 + (32 if boxes else 0) + (32 if quad else 0)}
+
+
+def box_records(mode, theta):
+    """Whether the build writes every node's box record (the per-target walk, the group walk
+    above theta = 1 or with buckets)."""
+    return (mode.get("walk") != 1 or theta > 1.0 or (mode.get("leaf_size") or 1) > 1)
 
 
 HBM_PEAK_GBS = 8000.0
@@ -259,7 +268,7 @@ def batched(glx, Pool, systems, eps, theta, dt, steps, warmup, mode, err_per_sys
         errs.append(rel_err(acc[tg], glx.direct_sum_f64(pos, m, 1.0, eps, tg.astype(np.uint32))))
     walk_ms = phases["walk"]
     bytes_tab = phase_bytes(mode.get("morton_bits"), mode.get("radix_bits"),
-                            mode.get("multipole") == 2)
+                            mode.get("multipole") == 2, box_records(mode, theta))
     flops = (FLOPS_POINT * cnt["point_terms"] + FLOPS_QUAD * cnt["quad_terms"] +
              FLOPS_M2L * cnt["far_nodes"] + (FLOPS_L2P * n_tot if mode.get("far_theta") else 0))
     pool.close()
@@ -325,7 +334,7 @@ def single(glx, pos, vel, mass, eps, theta, dt, steps, warmup, mode, d64_targets
     flops = (FLOPS_POINT * c0["point_terms"] + FLOPS_QUAD * c0["quad_terms"] +
              FLOPS_M2L * c0["far_nodes"] + (FLOPS_L2P * n if mode.get("far_theta") else 0))
     bytes_tab = phase_bytes(mode.get("morton_bits"), mode.get("radix_bits"),
-                            mode.get("multipole") == 2)
+                            mode.get("multipole") == 2, box_records(mode, theta))
     sysm.close()
     return {
         "snap": snap, "acc0": acc0,
