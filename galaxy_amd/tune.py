@@ -149,7 +149,7 @@ def _phase_ms(systems, knobs, mode, softening, theta, dt, steps, phase):
 def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e-3,
          p99_bound=1e-2, margin=0.8, per_system=32, seed=5, leaf_sizes=None, far_thetas=None,
          mono_ratios=None, group_shapes=None, radix_widths=None, radix_systems=None,
-         key_widths=None, width_margin=0.02):
+         key_widths=None, width_margin=0.02, shape_margin=0.01):
     """All knobs, in four stages on the sample `systems`:
     1. leaf size x far_theta x mono_ratio at the default block shape and radix width
        (`tune_leaf_size`), with 63-bit keys;
@@ -158,7 +158,10 @@ def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e
        error-checked; the 30-bit (reference-exact) build is kept unless the 63-bit one
        is more than `width_margin` faster;
     3. block shape (group_size, group_lanes) at the chosen knobs and width: each shape is
-       timed by its walk phase and error-checked;
+       timed by its walk phase and error-checked; the first shape listed (the default,
+       64 bodies with up to 4 lanes each) is kept unless another within the error bounds
+       is more than `shape_margin` faster (shapes a per cent apart on the sample swap
+       places from run to run);
     4. radix width at the chosen knobs, key width and shape, timed by the sort phase on
        `radix_systems` (default `systems`; the whole workload where it is at hand: the
        sort's cost per key changes with the number of tiles in flight, and a sample
@@ -198,7 +201,10 @@ def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e
         pool.close()
         shape_table.append({"group_size": gs, "group_lanes": lanes, "walk_ms": ms,
                             "rate": n / (ms * 1e-3), "median_err": med, "p99_err": p99})
-    shape = _best(shape_table, err_bound, margin, p99_bound)
+    ok = [t for t in shape_table
+          if t["median_err"] <= err_bound * margin and t["p99_err"] <= p99_bound] or shape_table
+    fastest = max(ok, key=lambda t: t["rate"])
+    shape = ok[0] if ok[0]["rate"] * (1.0 + shape_margin) >= fastest["rate"] else fastest
     knobs.update(group_size=shape["group_size"], group_lanes=shape["group_lanes"])
     radix_table = []
     for bits in (radix_widths or RADIX_WIDTHS):

@@ -247,7 +247,9 @@ def test_autotuner_block_shape_and_radix_width():
     best = next(t for t in r["shape_table"]
                 if (t["group_size"], t["group_lanes"]) == (r["group_size"], r["group_lanes"]))
     assert best in ok
-    assert_tuner_pick(best, r["shape_table"])
+    # the first shape within the bounds unless another is more than 1% faster
+    fastest = max(t["rate"] for t in ok)
+    assert best is ok[0] if ok[0]["rate"] * 1.01 >= fastest else best["rate"] == fastest
     assert r["radix_bits"] == min(r["radix_table"], key=lambda t: t["sort_ms"])["radix_bits"]
     import tempfile, os
     with tempfile.TemporaryDirectory() as d:
