@@ -66,8 +66,8 @@ FLOPS_L2P = 140
 # CU-cycle (four SIMDs each taking four cycles per wave64 instruction) is the issue limit,
 # so the walk is VALU-issue bound at whatever clock the GPU runs.
 WALK_COUNTERS = {
-    (1, 0.6, 0.9, 64, 4, 30): {"traffic": 4.660e9 + 0.684e9, "valu_per_cu_cycle": 1.02},
-    (1, 0.65, 0.8, 64, 4, 30): {"traffic": 4.970e9 + 0.679e9, "valu_per_cu_cycle": 1.02},
+    (1, 0.6, 0.9, 64, 4, 30): {"traffic": 4.641e9 + 0.684e9, "valu_per_cu_cycle": 1.02},
+    (1, 0.65, 0.8, 64, 4, 30): {"traffic": 4.951e9 + 0.679e9, "valu_per_cu_cycle": 1.02},
 }
 # algorithmic bytes per particle for the HBM-bound phases (per force evaluation)
 
