@@ -743,7 +743,18 @@ def main():
         pos, vel, mass, _ = ic.collision(n, args.seed)
         tg = np.sort(np.random.default_rng(11).choice(n, min(n, 4096), replace=False))
         tg = tg.astype(np.uint32)
-        f = single(glx, pos, vel, mass, ce, ct, cdt, args.collision_steps, 2, fast_mode, tg)
+        col_mode = dict(fast_mode)
+        if not args.radix_bits:  # the digit width for this system's 63-bit keys
+            from galaxy_amd.tune import tune_radix
+
+            cknobs = {k: col_mode.pop(k) for k in ("leaf_size", "far_theta", "mono_ratio",
+                                                    "group_size", "group_lanes", "radix_bits")
+                      if k in col_mode}
+            col_mode["radix_bits"], radix_table = tune_radix(
+                [(pos, vel, mass)], cknobs, softening=ce, theta=ct, dt=cdt, mode=col_mode)
+            col_mode.update({k: v for k, v in cknobs.items() if k != "radix_bits"})
+            log(f"collision radix width {col_mode['radix_bits']} bits: {radix_table}")
+        f = single(glx, pos, vel, mass, ce, ct, cdt, args.collision_steps, 2, col_mode, tg)
         log(f"collision fast: {f['value']:.4g} pf/s")
         r = single(glx, pos, vel, mass, ce, ct, cdt, 1, 1, ref_mode, tg)
         log(f"collision reference build: {r['value']:.4g} pf/s")
@@ -755,6 +766,9 @@ def main():
         out["collision_2p20"] = {
             "workload": "two-galaxy collision, plain single-system resident path",
             "n": n, "theta": ct, "eps": ce, "dt": cdt,
+            "build": {k: col_mode.get(k) for k in ("morton_bits", "leaf_size", "far_theta",
+                                                   "mono_ratio", "group_size", "group_lanes",
+                                                   "radix_bits")},
             "fast": strip(f),
             "reference_build": strip(r),
         }

@@ -146,6 +146,22 @@ def _phase_ms(systems, knobs, mode, softening, theta, dt, steps, phase):
     return pool, ids, ms
 
 
+def tune_radix(systems, knobs, *, softening, theta, dt=1e-3, mode=None, steps=3,
+               radix_widths=None):
+    """The sort's digit width for `systems` at the given knobs: each width (default 8, 9
+    and 10 bits) timed by the sort phase of a pool of the systems. The sorted order does
+    not depend on it, so there is no error check; the key width in `mode` sets the number
+    of passes (30-bit keys: 4, 4, 3; 63-bit keys: 8, 7, 7). Returns (width, table)."""
+    mode = dict(mode or dict(morton_bits=63, walk=lbvh.WALK_GROUP, multipole=2))
+    table = []
+    for bits in (radix_widths or RADIX_WIDTHS):
+        pool, _, ms = _phase_ms(systems, dict(knobs, radix_bits=bits), mode, softening, theta,
+                                dt, steps, "sort")
+        pool.close()
+        table.append({"radix_bits": bits, "sort_ms": ms})
+    return min(table, key=lambda row: row["sort_ms"])["radix_bits"], table
+
+
 def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e-3,
          p99_bound=1e-2, margin=0.8, per_system=32, seed=5, leaf_sizes=None, far_thetas=None,
          mono_ratios=None, group_shapes=None, radix_widths=None, radix_systems=None,
@@ -206,14 +222,10 @@ def tune(systems, *, softening, theta, dt=1e-3, mode=None, steps=3, err_bound=1e
     fastest = max(ok, key=lambda t: t["rate"])
     shape = ok[0] if ok[0]["rate"] * (1.0 + shape_margin) >= fastest["rate"] else fastest
     knobs.update(group_size=shape["group_size"], group_lanes=shape["group_lanes"])
-    radix_table = []
-    for bits in (radix_widths or RADIX_WIDTHS):
-        pool, _, ms = _phase_ms(radix_systems or systems, dict(knobs, radix_bits=bits), mode,
-                                softening, theta, dt, steps, "sort")
-        pool.close()
-        radix_table.append({"radix_bits": bits, "sort_ms": ms})
-    radix = min(radix_table, key=lambda row: row["sort_ms"])
-    out = dict(knobs, morton_bits=mode["morton_bits"], radix_bits=radix["radix_bits"])
+    radix_bits, radix_table = tune_radix(radix_systems or systems, knobs, softening=softening,
+                                         theta=theta, dt=dt, mode=mode, steps=steps,
+                                         radix_widths=radix_widths)
+    out = dict(knobs, morton_bits=mode["morton_bits"], radix_bits=radix_bits)
     out.update(table=r["table"], width_table=width_table, shape_table=shape_table,
                radix_table=radix_table)
     return out
